@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-timestep solver loop on one MI355X.
+
+Workload (default): 400x400x400 voxels x 8 substrates, FP32, dx = 20 um, dt = 0.01; 1,000,000 agents in
+16 Gaussian clusters (sigma 400 um); Dirichlet conditions on all 6 boundary faces and on 1% of the voxels;
+every step is secretion/uptake -> x/y/z Thomas sweeps with Dirichlet -> gradient, with the agent
+reduction recomputed every 10th step (the reference driver's pattern, src/main.cpp:100).  Synthetic data
+from a fixed seed.  Each step is one replay of the captured HIP graph; inputs are resident in HBM when the
+timed region starts.
+
+Prints ONE JSON line: throughput in voxel*substrate updates/s, the roofline of the dominant kernel (HIP
+events on the solver's stream) and the CPU baseline (the oracle, a restatement of the reference's host
+OpenMP path, timed on this machine's cores on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "voxel·substrate updates/s + % HBM peak: batched BF16-store vs plain FP32"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(cfg):
+    """Bytes each kernel must move per launch (what a fused, single-pass design reads and writes once)."""
+    X, Y, Z = cfg["grid"]
+    S = cfg["S"]
+    VS = X * Y * Z * S
+    A = cfg["agents"]["volumes"].shape[0]
+    n_int = 0 if cfg["interior"] is None else cfg["interior"][0].size // 3
+    face_elems = 2 * (X * Y + Y * Z + X * Z) * S if cfg["faces"] else 0
+    per = {
+        "sweep_x": 8 * VS, "sweep_y": 8 * VS, "sweep_z": 8 * VS,  # read + write every density once
+        "gradient": 16 * VS,  # read densities, write 3 components
+        "dirichlet_interior": n_int * (8 + 9 * S),  # voxel index, values, conditions, density write
+        "dirichlet_faces": face_elems * 4,
+        # positions + volume, then per substrate: reduced sums (12 B), internalized read/write (8 B),
+        # density read/write (8 B)
+        "secretion": A * (16 + 28 * S),
+    }
+    # one step of the fused design: 3 sweeps (24 B) + gradient written from the last sweep (12 B) per
+    # voxel*substrate, Dirichlet interior after each of the 3 sweeps and before the first, agents
+    step = 36 * VS + 4 * per["dirichlet_interior"] + per["dirichlet_faces"] + per["secretion"]
+    return per, step
+
+
+def cpu_baseline(cfg, budget_s, max_steps):
+    """The oracle (restated host OpenMP path, float build) on the same workload; time whole steps until the
+    budget is spent.  Returns the JSON object."""
+    from oracle import pyoracle
+    from parabiofvm_amd.configs import apply_config
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    pyoracle.build()
+    o = pyoracle.Oracle(cfg["dims"], cfg["mins"], cfg["maxs"], cfg["voxel_shape"], cfg["S"], cfg["dt"], cfg["D"],
+                        cfg["decay"], np.zeros(cfg["S"], dtype=np.float32), dtype=np.float32, threads=threads)
+    apply_config(cfg, o)
+    o.initialize()
+    times = []
+    phases = []
+    t_start = time.perf_counter()
+    i = 0
+    while i < max_steps and (time.perf_counter() - t_start) < budget_s:
+        t0 = time.perf_counter()
+        ph = o.timestep(i % 10 == 0, True)
+        times.append(time.perf_counter() - t0)
+        phases.append(ph)
+        i += 1
+    VS = cfg["updates_per_step"]
+    total = sum(times)
+    cpu_model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    ph = np.array(phases)
+    return {
+        "value": VS * len(times) / total, "unit": "voxel·substrate updates/s", "cores": threads, "kind": "port",
+        "sample": (f"{len(times)} full timesteps (first with the agent recompute) of the same workload, "
+                   f"{total:.1f} s; CPU oracle restating the reference host OpenMP path, float build, "
+                   f"-O3 -fopenmp -ffp-contract=off, {threads} threads on {cpu_model}"),
+        "ms_per_step": 1e3 * total / len(times),
+        "phase_ms": {"secretion": float(ph[:, 0].mean()), "diffusion": float(ph[:, 1].mean()),
+                     "gradient": float(ph[:, 2].mean())},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--algorithm", choices=["partitioned", "exact"], default="partitioned")
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--kernel-iters", type=int, default=20)
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline stepping")
+    ap.add_argument("--cpu-max-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--shape", type=int, nargs=3, default=None, help="override the mesh (smaller runs)")
+    ap.add_argument("--agents", type=int, default=None)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (each GPU box has one MI355X); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import parabiofvm_amd
+    from parabiofvm_amd.configs import apply_config, make_config
+
+    t = time.perf_counter()
+    cfg = make_config(1, shape=args.shape, agents=args.agents)
+    X, Y, Z = cfg["grid"]
+    S = cfg["S"]
+    log(f"[bench] workload generated in {time.perf_counter() - t:.1f}s: {X}x{Y}x{Z} x {S} substrates, "
+        f"{cfg['agents']['volumes'].shape[0]} agents")
+
+    gpu = parabiofvm_amd.DeviceSolver(cfg["dims"], cfg["mins"], cfg["maxs"], cfg["voxel_shape"], S, cfg["dt"],
+                                      cfg["D"], cfg["decay"], np.zeros(S, dtype=np.float32), device=0)
+    apply_config(cfg, gpu)
+    gpu.initialize()
+    alg = parabiofvm_amd.BFVM_SWEEP_PARTITIONED if args.algorithm == "partitioned" else parabiofvm_amd.BFVM_SWEEP_EXACT
+    gpu.set_sweep_algorithm(alg, args.chunk)
+    gpu.store()
+    log(f"[bench] initialized; partitions {[gpu.partition(a) for a in range(3)]}")
+
+    for i in range(args.warmup):
+        gpu.timestep(i % 10 == 0, True)
+    gpu.wait()
+    log(f"[bench] {args.warmup} warmup steps done")
+
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        gpu.timestep(i % 10 == 0, True)
+    gpu.wait()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    VS = cfg["updates_per_step"]
+    value = VS * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    log(f"[bench] {args.steps} steps in {elapsed:.3f}s -> {ms_per_step:.3f} ms/step, {value:.3e} updates/s")
+
+    # per-kernel averages (HIP events around back-to-back launches on the solver's stream)
+    per_bytes, step_bytes = algorithmic_bytes(cfg)
+    kernel_ms = {}
+    for k in ("sweep_x", "sweep_y", "sweep_z", "gradient", "dirichlet_interior", "dirichlet_faces", "secretion",
+              "secretion_recompute"):
+        kernel_ms[k] = gpu.benchmark(k, args.kernel_iters)
+        log(f"[bench] {k}: {kernel_ms[k]:.4f} ms/launch")
+    launches_per_step = {"sweep_x": 1, "sweep_y": 1, "sweep_z": 1, "gradient": 1, "dirichlet_interior": 4,
+                         "dirichlet_faces": 1, "secretion": 0.9, "secretion_recompute": 0.1}
+    share = {k: kernel_ms[k] * launches_per_step[k] for k in kernel_ms}
+    dominant = max(("sweep_x", "sweep_y", "sweep_z", "gradient"), key=lambda k: share[k])
+    achieved = per_bytes[dominant] / (kernel_ms[dominant] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": per_bytes[dominant], "avg_launch_ms": round(kernel_ms[dominant], 4)}
+
+    result = {
+        "metric": METRIC, "value": value, "unit": "voxel·substrate updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": (f"single instance {X}x{Y}x{Z} voxels x {S} substrates, FP32 storage, "
+                                f"{cfg['agents']['volumes'].shape[0]} agents in 16 clusters, 6 Dirichlet faces + 1% "
+                                f"interior Dirichlet voxels, gradient every step, agent recompute every 10 steps"),
+                   "grid": [X, Y, Z], "substrates": S, "agents": int(cfg["agents"]["volumes"].shape[0]),
+                   "sweep": args.algorithm, "partitions": [list(gpu.partition(a)) for a in range(3)]},
+        "hbm_frac_step": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "algorithmic_bytes_per_step": step_bytes,
+        "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
+        "roofline": roofline,
+    }
+    gpu.close()
+
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle) ...")
+        result["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget, args.cpu_max_steps)
+        log(f"[bench] CPU baseline: {result['cpu_baseline']['ms_per_step']:.1f} ms/step")
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

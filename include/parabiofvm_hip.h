@@ -1,0 +1,203 @@
+/*
+ * parabiofvm_hip.h — C ABI of the MI355X (gfx950) solver library libparabiofvm_hip.so.
+ *
+ * One bfvm_ctx is the device side of one microenvironment: it owns every device
+ * buffer and one HIP stream (default priority).  The entry points are what the
+ * reference's device solver facade does per call, flattened to plain pointers
+ * and sizes (no C++ or torch types):
+ *
+ *   reference (asmelko/paraBioFVM)                                 entry point
+ *   -------------------------------------------------------------  ---------------------------
+ *   device_context() ctor, two streams   CUDA/device_context.cpp:7-11    bfvm_create
+ *   ~device_context                      CUDA/device_context.cpp:13-17   bfvm_destroy
+ *   solver::initialize                   CUDA/solver.cpp:42-48,
+ *     (device_context / cell / diffusion / dirichlet initialize:
+ *      device_context.cpp:19-26, cell_solver.cpp:329-340,
+ *      diffusion_solver.cpp:41-103, dirichlet_solver.cpp:17-88)         bfvm_initialize
+ *   solver::store_data_to_solver         CUDA/solver.cpp:50,
+ *     device_context::copy_to_device     device_context.cpp:56-97       bfvm_store_densities,
+ *                                                                        bfvm_store_agents
+ *   solver::load_data_from_solver        CUDA/solver.cpp:52,
+ *     device_context::copy_to_host       device_context.cpp:99-116      bfvm_load_densities,
+ *                                                                        bfvm_load_internalized,
+ *                                                                        bfvm_load_gradients
+ *   diffusion_solver::solve              host/diffusion_solver.cpp:60-70,
+ *                                        CUDA/diffusion_solver.cpp:128-152 bfvm_diffusion_solve
+ *   dirichlet_solver::solve              host/dirichlet_solver.cpp:112-120 bfvm_dirichlet_solve
+ *   gradient_solver::solve (host-only in the reference device facade,
+ *                                        host/gradient_solver.cpp:275-291) bfvm_gradient_solve
+ *   cell_solver::simulate_secretion_and_uptake
+ *                                        host/cell_solver.cpp:272-299,
+ *                                        CUDA/cell_solver.cpp:249-286      bfvm_cell_simulate
+ *   cell_solver::release_internalized_substrates
+ *                                        host/cell_solver.cpp:340-346,
+ *                                        CUDA/cell_solver.cpp:288-294      bfvm_cell_release
+ *   solver::wait_for_all                 CUDA/solver.cpp:54-58             bfvm_wait_for_all
+ *   one timestep of main_device.cpp:70-111 (secretion, diffusion, gradient)
+ *                                        as a captured HIP graph            bfvm_timestep
+ *
+ * Every function returns 0 on success and a negative status on error; the
+ * message of the last error on the calling thread is bfvm_last_error().  No
+ * entry point aborts the process.  All enqueued work runs on the context's
+ * stream in call order; bfvm_load_* and bfvm_wait_for_all synchronise it.
+ *
+ * Arrays follow the reference's layouts (src/traits.h:65-93): densities
+ * [z][y][x][s]; gradients [d][z][y][x][s] for 3-D meshes and [y][x][s][d] for
+ * 1-D/2-D meshes; agent arrays [agent][s]; positions [agent][dims]; the interior
+ * Dirichlet list is [voxel][dims] int32 with values/conditions [voxel][s].
+ * Boolean arrays are one byte per entry (C++ bool).
+ */
+#ifndef PARABIOFVM_HIP_H
+#define PARABIOFVM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bfvm_ctx bfvm_ctx;
+
+/* Status codes. */
+#define BFVM_OK 0
+#define BFVM_ERR_ARGUMENT (-1)
+#define BFVM_ERR_STATE (-2)
+#define BFVM_ERR_HIP (-3)
+#define BFVM_ERR_UNSUPPORTED (-4)
+
+/* Sweep algorithm (bfvm_set_sweep_algorithm). */
+#define BFVM_SWEEP_PARTITIONED 0 /* register-resident partitioned (modified) Thomas, FP64-derived coefficients */
+#define BFVM_SWEEP_EXACT 1		 /* one thread per line, the reference's FP32 coefficients and operation order */
+
+/*
+ * Microenvironment description: the fields of `microenvironment` (include/BioFVM/microenvironment.h:19-63)
+ * and `cartesian_mesh` (include/BioFVM/mesh.h:7-29) that the solvers read.  Pointers are read during
+ * bfvm_initialize only.  A NULL boundary-values pointer means "no condition on that face"
+ * (microenvironment.cpp:106-109 default).
+ */
+typedef struct bfvm_microenvironment_desc
+{
+	int32_t dims; /* 1, 2 or 3 */
+	int32_t bounding_box_mins[3];
+	int32_t bounding_box_maxs[3];
+	int32_t voxel_shape[3];
+	int32_t substrates_count;
+	float diffusion_time_step;
+	const float* diffusion_coefficients; /* [S] */
+	const float* decay_rates;			 /* [S] */
+
+	const float* dirichlet_min_boundary_values[3];		   /* [S] or NULL, per axis */
+	const float* dirichlet_max_boundary_values[3];		   /* [S] or NULL */
+	const uint8_t* dirichlet_min_boundary_conditions[3]; /* [S] */
+	const uint8_t* dirichlet_max_boundary_conditions[3]; /* [S] */
+
+	int32_t dirichlet_interior_voxels_count;
+	const int32_t* dirichlet_interior_voxels;		 /* [count][dims] */
+	const float* dirichlet_interior_values;		 /* [count][S] */
+	const uint8_t* dirichlet_interior_conditions; /* [count][S] */
+
+	int32_t compute_internalized_substrates;
+} bfvm_microenvironment_desc;
+
+/* Agent SoA (include/BioFVM/agent_data.h:11-34). fraction_released_at_death may be NULL when unused. */
+typedef struct bfvm_agents_desc
+{
+	int32_t agents_count;
+	const float* secretion_rates;			 /* [A][S] */
+	const float* saturation_densities;		 /* [A][S] */
+	const float* uptake_rates;				 /* [A][S] */
+	const float* net_export_rates;			 /* [A][S] */
+	const float* internalized_substrates;	 /* [A][S] */
+	const float* fraction_released_at_death; /* [A][S] or NULL */
+	const float* volumes;					 /* [A] */
+	const float* positions;					 /* [A][dims] */
+} bfvm_agents_desc;
+
+/* Device addresses of the context's buffers, for harnesses that keep data resident in HBM. */
+typedef struct bfvm_device_buffers
+{
+	float* densities;			   /* [V*S] */
+	float* gradients;			   /* [dims*V*S] */
+	float* internalized_substrates; /* [capacity*S] */
+	int64_t voxel_count;
+	int32_t grid_shape[3];
+	int32_t substrates_count;
+	void* stream; /* hipStream_t the context enqueues on */
+} bfvm_device_buffers;
+
+const char* bfvm_last_error(void);
+
+/* device_context ctor: selects `device`, creates the context's stream. */
+int bfvm_create(int32_t device, bfvm_ctx** out);
+int bfvm_destroy(bfvm_ctx* ctx);
+
+/* solver::initialize: allocates densities (filled with zeros) and gradients, uploads the Dirichlet
+ * description, computes the Thomas coefficients on the GPU.  May be called again to re-initialise. */
+int bfvm_initialize(bfvm_ctx* ctx, const bfvm_microenvironment_desc* desc);
+
+/* Selects the sweep algorithm (default BFVM_SWEEP_PARTITIONED) and, for the partitioned one, the
+ * maximum chunk length per thread (0 = automatic).  Takes effect at the next solve. */
+int bfvm_set_sweep_algorithm(bfvm_ctx* ctx, int32_t algorithm, int32_t chunk);
+
+/* store_data_to_solver / load_data_from_solver parts.  Host pointers, synchronous w.r.t. the host buffer. */
+int bfvm_store_densities(bfvm_ctx* ctx, const float* densities);
+int bfvm_store_agents(bfvm_ctx* ctx, const bfvm_agents_desc* agents);
+int bfvm_load_densities(bfvm_ctx* ctx, float* densities);
+int bfvm_load_gradients(bfvm_ctx* ctx, float* gradients);
+int bfvm_load_internalized(bfvm_ctx* ctx, float* internalized_substrates);
+
+/* Solvers (asynchronous on the context's stream). */
+int bfvm_diffusion_solve(bfvm_ctx* ctx);
+int bfvm_dirichlet_solve(bfvm_ctx* ctx);
+int bfvm_gradient_solve(bfvm_ctx* ctx);
+int bfvm_cell_simulate(bfvm_ctx* ctx, int32_t recompute);
+int bfvm_cell_release(bfvm_ctx* ctx, int32_t agent_index);
+
+/* One timestep, secretion -> diffusion (Dirichlet, x, Dirichlet, y, Dirichlet, z, Dirichlet) -> gradient,
+ * replayed from a captured HIP graph (captured on first use per (recompute, with_gradient) pair). */
+int bfvm_timestep(bfvm_ctx* ctx, int32_t recompute, int32_t with_gradient);
+
+int bfvm_wait_for_all(bfvm_ctx* ctx);
+
+int bfvm_get_device_buffers(bfvm_ctx* ctx, bfvm_device_buffers* out);
+
+/* Device-resident variants for harnesses: same as the host ones but from/to device memory on the
+ * context's stream (asynchronous). */
+int bfvm_store_densities_device(bfvm_ctx* ctx, const float* device_densities);
+
+/*
+ * Diagnostics: the per-axis partition the partitioned sweep uses (blocks, block length) and the
+ * reference-order FP32 Thomas coefficients b' [n][S], c [S], e [n-1][S] computed on the GPU
+ * (host/diffusion_solver.cpp:72-142).
+ */
+int bfvm_sweep_partition(bfvm_ctx* ctx, int32_t axis, int32_t* blocks, int32_t* block_length);
+int bfvm_exact_coefficients(bfvm_ctx* ctx, int32_t axis, float* b, float* c, float* e);
+
+/* mesh::voxel_position<dims> (src/mesh.cpp:47-53) for `count` positions [count][dims], computed by the GPU
+ * kernel the secretion solver uses; out3 gets [count][3] indices, -1 on axes where the point is outside. */
+int bfvm_voxel_positions(bfvm_ctx* ctx, int32_t count, const float* positions, int32_t* out3);
+
+/*
+ * Measurement (no counterpart in the reference, whose only timing is std::chrono around whole phases,
+ * src/main.cpp:73-101): enqueues `iterations` back-to-back launches of one kernel or phase on the context's
+ * stream between two HIP events and returns the average milliseconds per launch.  The work runs on the
+ * context's real data (densities are modified).
+ */
+#define BFVM_KERNEL_SWEEP_X 0
+#define BFVM_KERNEL_SWEEP_Y 1
+#define BFVM_KERNEL_SWEEP_Z 2
+#define BFVM_KERNEL_GRADIENT 3
+#define BFVM_KERNEL_DIRICHLET_INTERIOR 4
+#define BFVM_KERNEL_DIRICHLET_FACES 5
+#define BFVM_PHASE_SECRETION 10		   /* cell solver, recompute = false */
+#define BFVM_PHASE_SECRETION_RECOMPUTE 11 /* cell solver, recompute = true */
+#define BFVM_PHASE_DIFFUSION 12
+#define BFVM_PHASE_GRADIENT 13
+#define BFVM_PHASE_TIMESTEP 14 /* graph replay, recompute = false, with gradient */
+int bfvm_benchmark(bfvm_ctx* ctx, int32_t what, int32_t iterations, float* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

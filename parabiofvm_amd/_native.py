@@ -1,0 +1,130 @@
+"""ctypes binding of libparabiofvm_hip.so (C ABI declared in include/parabiofvm_hip.h).
+
+The library is built in-tree (parabiofvm_amd/libparabiofvm_hip.so, see
+parabiofvm_amd/csrc/Makefile).  Loading fails loudly when it is missing: there
+is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libparabiofvm_hip.so")
+
+BFVM_SWEEP_PARTITIONED = 0
+BFVM_SWEEP_EXACT = 1
+
+
+class BfvmError(RuntimeError):
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn} failed ({code}): {msg}")
+        self.code = code
+
+
+_F = ctypes.POINTER(ctypes.c_float)
+_U8 = ctypes.POINTER(ctypes.c_uint8)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+
+
+class MicroenvironmentDesc(ctypes.Structure):
+    _fields_ = [
+        ("dims", ctypes.c_int32),
+        ("bounding_box_mins", ctypes.c_int32 * 3),
+        ("bounding_box_maxs", ctypes.c_int32 * 3),
+        ("voxel_shape", ctypes.c_int32 * 3),
+        ("substrates_count", ctypes.c_int32),
+        ("diffusion_time_step", ctypes.c_float),
+        ("diffusion_coefficients", _F),
+        ("decay_rates", _F),
+        ("dirichlet_min_boundary_values", _F * 3),
+        ("dirichlet_max_boundary_values", _F * 3),
+        ("dirichlet_min_boundary_conditions", _U8 * 3),
+        ("dirichlet_max_boundary_conditions", _U8 * 3),
+        ("dirichlet_interior_voxels_count", ctypes.c_int32),
+        ("dirichlet_interior_voxels", _I32),
+        ("dirichlet_interior_values", _F),
+        ("dirichlet_interior_conditions", _U8),
+        ("compute_internalized_substrates", ctypes.c_int32),
+    ]
+
+
+class AgentsDesc(ctypes.Structure):
+    _fields_ = [
+        ("agents_count", ctypes.c_int32),
+        ("secretion_rates", _F),
+        ("saturation_densities", _F),
+        ("uptake_rates", _F),
+        ("net_export_rates", _F),
+        ("internalized_substrates", _F),
+        ("fraction_released_at_death", _F),
+        ("volumes", _F),
+        ("positions", _F),
+    ]
+
+
+class DeviceBuffers(ctypes.Structure):
+    _fields_ = [
+        ("densities", ctypes.c_void_p),
+        ("gradients", ctypes.c_void_p),
+        ("internalized_substrates", ctypes.c_void_p),
+        ("voxel_count", ctypes.c_int64),
+        ("grid_shape", ctypes.c_int32 * 3),
+        ("substrates_count", ctypes.c_int32),
+        ("stream", ctypes.c_void_p),
+    ]
+
+
+# name -> (restype, argtypes); every exported symbol of include/parabiofvm_hip.h
+PROTOTYPES = {
+    "bfvm_last_error": (ctypes.c_char_p, []),
+    "bfvm_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "bfvm_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "bfvm_initialize": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MicroenvironmentDesc)]),
+    "bfvm_set_sweep_algorithm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "bfvm_store_densities": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "bfvm_store_densities_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "bfvm_store_agents": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(AgentsDesc)]),
+    "bfvm_load_densities": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "bfvm_load_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "bfvm_load_internalized": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "bfvm_diffusion_solve": (ctypes.c_int, [ctypes.c_void_p]),
+    "bfvm_dirichlet_solve": (ctypes.c_int, [ctypes.c_void_p]),
+    "bfvm_gradient_solve": (ctypes.c_int, [ctypes.c_void_p]),
+    "bfvm_cell_simulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "bfvm_cell_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "bfvm_timestep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "bfvm_wait_for_all": (ctypes.c_int, [ctypes.c_void_p]),
+    "bfvm_get_device_buffers": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DeviceBuffers)]),
+    "bfvm_sweep_partition": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32, _I32]),
+    "bfvm_exact_coefficients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p]),
+    "bfvm_voxel_positions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
+    "bfvm_benchmark": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _F]),
+}
+
+KERNELS = {"sweep_x": 0, "sweep_y": 1, "sweep_z": 2, "gradient": 3, "dirichlet_interior": 4, "dirichlet_faces": 5,
+           "secretion": 10, "secretion_recompute": 11, "diffusion": 12, "gradient_phase": 13, "timestep": 14}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises if it has not been built (no fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not found: build it with __graft_entry__.build() "
+                          f"(make -C parabiofvm_amd/csrc)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in PROTOTYPES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(fn, code):
+    if code != 0:
+        msg = load().bfvm_last_error()
+        raise BfvmError(fn, code, msg.decode() if msg else "")

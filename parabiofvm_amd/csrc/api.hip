@@ -1,0 +1,793 @@
+// C ABI of libparabiofvm_hip.so (declared in include/parabiofvm_hip.h).
+#include <cstring>
+#include <stdexcept>
+#include <unordered_map>
+
+#include "bfvm_internal.hpp"
+
+namespace bfvm {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+template <typename F>
+static int guarded(const char* fn, F&& f)
+{
+	try
+	{
+		return f();
+	}
+	catch (const hip_failure& e)
+	{
+		set_error(std::string(fn) + ": HIP error '" + hipGetErrorString(e.err) + "' in " + e.what + " (" + e.file +
+				  ":" + std::to_string(e.line) + ")");
+		return BFVM_ERR_HIP;
+	}
+	catch (const std::invalid_argument& e)
+	{
+		set_error(std::string(fn) + ": " + e.what());
+		return BFVM_ERR_ARGUMENT;
+	}
+	catch (const std::logic_error& e)
+	{
+		set_error(std::string(fn) + ": " + e.what());
+		return BFVM_ERR_STATE;
+	}
+	catch (const std::exception& e)
+	{
+		set_error(std::string(fn) + ": " + e.what());
+		return BFVM_ERR_UNSUPPORTED;
+	}
+}
+
+template <typename T>
+static void dfree(T*& p)
+{
+	if (p)
+		BFVM_HIP(hipFree(p));
+	p = nullptr;
+}
+
+template <typename T>
+static void dalloc(T*& p, size_t count)
+{
+	dfree(p);
+	if (count > 0)
+		BFVM_HIP(hipMalloc(&p, sizeof(T) * count));
+}
+
+static void require_init(const bfvm_ctx* ctx)
+{
+	if (!ctx)
+		throw std::invalid_argument("null context");
+	if (!ctx->initialized)
+		throw std::logic_error("solver not initialized (call bfvm_initialize first)");
+}
+
+static void free_agents(bfvm_ctx* ctx)
+{
+	auto& a = ctx->ag;
+	for (float** p : { &a.sec, &a.sat, &a.upt, &a.nex, &a.intern, &a.frac, &a.vol, &a.pos, &a.num, &a.den, &a.fac,
+					   &a.rnum, &a.rden, &a.rfac })
+		dfree(*p);
+	a.capacity = 0;
+	a.count = 0;
+}
+
+static void free_graphs(bfvm_ctx* ctx)
+{
+	for (auto& g : ctx->graphs)
+	{
+		if (g.exec)
+			BFVM_HIP(hipGraphExecDestroy(g.exec));
+		if (g.graph)
+			BFVM_HIP(hipGraphDestroy(g.graph));
+		g = graph_slot {};
+	}
+}
+
+static void free_fields(bfvm_ctx* ctx)
+{
+	dfree(ctx->dens);
+	dfree(ctx->grad);
+	dfree(ctx->d_D);
+	dfree(ctx->d_decay);
+	dfree(ctx->face_val);
+	dfree(ctx->face_cond);
+	dfree(ctx->int_vox);
+	dfree(ctx->int_val);
+	dfree(ctx->int_cond);
+	dfree(ctx->ballots);
+	dfree(ctx->conflict);
+	for (auto& t : ctx->axes)
+	{
+		dfree(t.exact_b);
+		dfree(t.exact_c);
+		dfree(t.exact_e);
+		dfree(t.coef);
+		dfree(t.red);
+		dfree(t.A);
+		t = axis_tables {};
+	}
+	ctx->n_interior = 0;
+	ctx->has_faces = false;
+	ctx->initialized = false;
+}
+
+static void enqueue_dirichlet(bfvm_ctx* ctx)
+{
+	launch_dirichlet_faces(ctx);
+	launch_dirichlet_interior(ctx);
+}
+
+// host/diffusion_solver.cpp:482-544: Dirichlet, x, Dirichlet, y, Dirichlet, z, Dirichlet.  The boundary
+// faces of the Dirichlet that follows each sweep are applied in that sweep's store epilogue.
+static void enqueue_diffusion(bfvm_ctx* ctx)
+{
+	enqueue_dirichlet(ctx);
+	for (int axis = 0; axis < ctx->mesh.dims; axis++)
+	{
+		launch_sweep(ctx, axis);
+		launch_dirichlet_interior(ctx);
+	}
+}
+
+static void check_gradient(const bfvm_ctx* ctx)
+{
+	for (int d = 0; d < ctx->mesh.dims; d++)
+		if (ctx->mesh.grid[d] < 2)
+			throw std::invalid_argument("gradient needs at least 2 voxels along every axis");
+}
+
+static void ensure_partitions(bfvm_ctx* ctx)
+{
+	for (int axis = 0; axis < ctx->mesh.dims; axis++)
+	{
+		build_partition(ctx, axis, ctx->chunk_request);
+		launch_partition_coefficients(ctx, axis);
+	}
+}
+
+} // namespace bfvm
+
+using namespace bfvm;
+
+extern "C" {
+
+const char* bfvm_last_error(void) { return g_error.c_str(); }
+
+int bfvm_create(int32_t device, bfvm_ctx** out)
+{
+	return guarded("bfvm_create", [&] {
+		if (!out)
+			throw std::invalid_argument("null output pointer");
+		*out = nullptr;
+		int count = 0;
+		BFVM_HIP(hipGetDeviceCount(&count));
+		if (device < 0 || device >= count)
+			throw std::invalid_argument("device index out of range");
+		BFVM_HIP(hipSetDevice(device));
+		auto* ctx = new bfvm_ctx();
+		ctx->device = device;
+		hipError_t err = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+		if (err != hipSuccess)
+		{
+			delete ctx;
+			BFVM_HIP(err);
+		}
+		*out = ctx;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_destroy(bfvm_ctx* ctx)
+{
+	if (!ctx)
+		return BFVM_OK;
+	return guarded("bfvm_destroy", [&] {
+		BFVM_HIP(hipSetDevice(ctx->device));
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		free_graphs(ctx);
+		free_agents(ctx);
+		free_fields(ctx);
+		BFVM_HIP(hipStreamDestroy(ctx->stream));
+		delete ctx;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_initialize(bfvm_ctx* ctx, const bfvm_microenvironment_desc* desc)
+{
+	return guarded("bfvm_initialize", [&] {
+		if (!ctx || !desc)
+			throw std::invalid_argument("null argument");
+		BFVM_HIP(hipSetDevice(ctx->device));
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		const int dims = desc->dims;
+		const int S = desc->substrates_count;
+		if (dims < 1 || dims > 3)
+			throw std::invalid_argument("dims must be 1, 2 or 3");
+		if (S < 1)
+			throw std::invalid_argument("substrates_count must be positive");
+		if (!desc->diffusion_coefficients || !desc->decay_rates)
+			throw std::invalid_argument("diffusion coefficients and decay rates are required");
+		free_graphs(ctx);
+		free_agents(ctx);
+		free_fields(ctx);
+
+		mesh_info m;
+		m.dims = dims;
+		m.S = S;
+		m.dt = desc->diffusion_time_step;
+		for (int d = 0; d < 3; d++)
+		{
+			m.mins[d] = desc->bounding_box_mins[d];
+			m.maxs[d] = desc->bounding_box_maxs[d];
+			m.vs[d] = desc->voxel_shape[d];
+			m.grid[d] = 1;
+		}
+		// mesh.cpp:7-25
+		for (int d = 0; d < dims; d++)
+		{
+			if (m.vs[d] <= 0)
+				throw std::invalid_argument("voxel_shape must be positive on every mesh axis");
+			m.grid[d] = (m.maxs[d] - m.mins[d] + m.vs[d] - 1) / m.vs[d];
+			if (m.grid[d] < 1)
+				throw std::invalid_argument("empty mesh axis");
+		}
+		m.V = (size_t)m.grid[0] * m.grid[1] * m.grid[2];
+		ctx->mesh = m;
+		ctx->D.assign(desc->diffusion_coefficients, desc->diffusion_coefficients + S);
+		ctx->decay.assign(desc->decay_rates, desc->decay_rates + S);
+		ctx->compute_internalized = desc->compute_internalized_substrates != 0;
+
+		const size_t VS = m.V * S;
+		dalloc(ctx->dens, VS);
+		BFVM_HIP(hipMemsetAsync(ctx->dens, 0, sizeof(float) * VS, ctx->stream));
+		ctx->grad_elems = VS * dims;
+		dalloc(ctx->grad, ctx->grad_elems);
+		BFVM_HIP(hipMemsetAsync(ctx->grad, 0, sizeof(float) * ctx->grad_elems, ctx->stream));
+		dalloc(ctx->d_D, S);
+		dalloc(ctx->d_decay, S);
+		BFVM_HIP(hipMemcpy(ctx->d_D, ctx->D.data(), sizeof(float) * S, hipMemcpyHostToDevice));
+		BFVM_HIP(hipMemcpy(ctx->d_decay, ctx->decay.data(), sizeof(float) * S, hipMemcpyHostToDevice));
+
+		// boundary faces (microenvironment.h:47-50)
+		std::vector<float> fval(6 * S, 0.f);
+		std::vector<uint8_t> fcond(6 * S, 0);
+		bool any = false;
+		for (int d = 0; d < dims; d++)
+			for (int mx = 0; mx < 2; mx++)
+			{
+				const float* v = mx ? desc->dirichlet_max_boundary_values[d] : desc->dirichlet_min_boundary_values[d];
+				const uint8_t* c =
+					mx ? desc->dirichlet_max_boundary_conditions[d] : desc->dirichlet_min_boundary_conditions[d];
+				if (!v)
+					continue;
+				if (!c)
+					throw std::invalid_argument("boundary values given without conditions");
+				for (int s = 0; s < S; s++)
+				{
+					fval[(2 * d + mx) * S + s] = v[s];
+					fcond[(2 * d + mx) * S + s] = c[s] ? 1 : 0;
+					any |= c[s] != 0;
+				}
+			}
+		ctx->has_faces = any;
+		dalloc(ctx->face_val, 6 * S);
+		dalloc(ctx->face_cond, 6 * S);
+		BFVM_HIP(hipMemcpy(ctx->face_val, fval.data(), sizeof(float) * 6 * S, hipMemcpyHostToDevice));
+		BFVM_HIP(hipMemcpy(ctx->face_cond, fcond.data(), 6 * S, hipMemcpyHostToDevice));
+
+		// interior list (microenvironment.h:42-45), one entry per voxel: a later entry overrides an earlier
+		// one for every substrate whose condition it sets (the reference applies the list in order)
+		const int64_t n_in = desc->dirichlet_interior_voxels_count;
+		if (n_in < 0)
+			throw std::invalid_argument("negative interior Dirichlet count");
+		if (n_in > 0 && (!desc->dirichlet_interior_voxels || !desc->dirichlet_interior_values ||
+						 !desc->dirichlet_interior_conditions))
+			throw std::invalid_argument("interior Dirichlet arrays missing");
+		std::vector<int64_t> vox;
+		std::vector<float> val;
+		std::vector<uint8_t> cond;
+		std::unordered_map<int64_t, size_t> where;
+		for (int64_t i = 0; i < n_in; i++)
+		{
+			int c[3] = { 0, 0, 0 };
+			for (int d = 0; d < dims; d++)
+			{
+				c[d] = desc->dirichlet_interior_voxels[i * dims + d];
+				if (c[d] < 0 || c[d] >= m.grid[d])
+					throw std::invalid_argument("interior Dirichlet voxel outside the mesh");
+			}
+			const int64_t lin = ((int64_t)c[2] * m.grid[1] + c[1]) * m.grid[0] + c[0];
+			auto it = where.find(lin);
+			size_t slot;
+			if (it == where.end())
+			{
+				slot = vox.size();
+				where.emplace(lin, slot);
+				vox.push_back(lin);
+				val.resize(val.size() + S, 0.f);
+				cond.resize(cond.size() + S, 0);
+			}
+			else
+				slot = it->second;
+			for (int s = 0; s < S; s++)
+				if (desc->dirichlet_interior_conditions[i * S + s])
+				{
+					val[slot * S + s] = desc->dirichlet_interior_values[i * S + s];
+					cond[slot * S + s] = 1;
+				}
+		}
+		ctx->n_interior = (int64_t)vox.size();
+		if (ctx->n_interior > 0)
+		{
+			dalloc(ctx->int_vox, vox.size());
+			dalloc(ctx->int_val, val.size());
+			dalloc(ctx->int_cond, cond.size());
+			BFVM_HIP(hipMemcpy(ctx->int_vox, vox.data(), sizeof(int64_t) * vox.size(), hipMemcpyHostToDevice));
+			BFVM_HIP(hipMemcpy(ctx->int_val, val.data(), sizeof(float) * val.size(), hipMemcpyHostToDevice));
+			BFVM_HIP(hipMemcpy(ctx->int_cond, cond.data(), cond.size(), hipMemcpyHostToDevice));
+		}
+
+		// cell solver state (host/cell_solver.cpp:366-373): ballots value-initialised to 0
+		dalloc(ctx->ballots, m.V);
+		BFVM_HIP(hipMemsetAsync(ctx->ballots, 0, sizeof(int) * m.V, ctx->stream));
+		dalloc(ctx->conflict, 1);
+		BFVM_HIP(hipMemsetAsync(ctx->conflict, 0, sizeof(int), ctx->stream));
+
+		// diffusion coefficients (host/diffusion_solver.cpp:29-58), computed on the GPU
+		for (int axis = 0; axis < dims; axis++)
+		{
+			auto& t = ctx->axes[axis];
+			const int n = m.grid[axis];
+			t.n = n;
+			dalloc(t.exact_b, (size_t)n * S);
+			if (n > 1)
+			{
+				dalloc(t.exact_c, S);
+				dalloc(t.exact_e, (size_t)(n - 1) * S);
+			}
+			launch_exact_coefficients(ctx, axis);
+		}
+		ensure_partitions(ctx);
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		ctx->initialized = true;
+		ctx->epoch++;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_set_sweep_algorithm(bfvm_ctx* ctx, int32_t algorithm, int32_t chunk)
+{
+	return guarded("bfvm_set_sweep_algorithm", [&] {
+		if (!ctx)
+			throw std::invalid_argument("null context");
+		if (algorithm != BFVM_SWEEP_PARTITIONED && algorithm != BFVM_SWEEP_EXACT)
+			throw std::invalid_argument("unknown sweep algorithm");
+		if (chunk < 0 || chunk > 64)
+			throw std::invalid_argument("chunk must be in [0, 64]");
+		BFVM_HIP(hipSetDevice(ctx->device));
+		ctx->algorithm = algorithm;
+		ctx->chunk_request = chunk;
+		if (ctx->initialized)
+		{
+			BFVM_HIP(hipStreamSynchronize(ctx->stream));
+			ensure_partitions(ctx);
+			BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		}
+		ctx->epoch++;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_store_densities(bfvm_ctx* ctx, const float* densities)
+{
+	return guarded("bfvm_store_densities", [&] {
+		require_init(ctx);
+		if (!densities)
+			throw std::invalid_argument("null densities");
+		BFVM_HIP(hipSetDevice(ctx->device));
+		BFVM_HIP(hipMemcpyAsync(ctx->dens, densities, sizeof(float) * ctx->mesh.V * ctx->mesh.S,
+								hipMemcpyHostToDevice, ctx->stream));
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_store_densities_device(bfvm_ctx* ctx, const float* device_densities)
+{
+	return guarded("bfvm_store_densities_device", [&] {
+		require_init(ctx);
+		if (!device_densities)
+			throw std::invalid_argument("null densities");
+		BFVM_HIP(hipSetDevice(ctx->device));
+		BFVM_HIP(hipMemcpyAsync(ctx->dens, device_densities, sizeof(float) * ctx->mesh.V * ctx->mesh.S,
+								hipMemcpyDeviceToDevice, ctx->stream));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_store_agents(bfvm_ctx* ctx, const bfvm_agents_desc* agents)
+{
+	return guarded("bfvm_store_agents", [&] {
+		require_init(ctx);
+		if (!agents || agents->agents_count < 0)
+			throw std::invalid_argument("bad agents description");
+		BFVM_HIP(hipSetDevice(ctx->device));
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		const size_t A = (size_t)agents->agents_count;
+		const size_t S = ctx->mesh.S;
+		const size_t dims = ctx->mesh.dims;
+		if (A > 0 && (!agents->secretion_rates || !agents->saturation_densities || !agents->uptake_rates ||
+					  !agents->net_export_rates || !agents->internalized_substrates || !agents->volumes ||
+					  !agents->positions))
+			throw std::invalid_argument("agent arrays missing");
+		auto& a = ctx->ag;
+		// device_context.cpp:28-54 / cell_solver.cpp:296-327: reallocate on capacity change
+		if (A > a.capacity)
+		{
+			free_agents(ctx);
+			a.capacity = A;
+			for (float** p : { &a.sec, &a.sat, &a.upt, &a.nex, &a.intern, &a.frac, &a.num, &a.den, &a.fac, &a.rnum,
+							   &a.rden, &a.rfac })
+			{
+				dalloc(*p, A * S);
+				BFVM_HIP(hipMemsetAsync(*p, 0, sizeof(float) * A * S, ctx->stream));
+			}
+			dalloc(a.vol, A);
+			dalloc(a.pos, A * dims);
+		}
+		if ((int32_t)A != a.count)
+			ctx->epoch++;
+		a.count = (int32_t)A;
+		if (A > 0)
+		{
+			auto up = [&](float* dst, const float* src, size_t n) {
+				BFVM_HIP(hipMemcpyAsync(dst, src, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+			};
+			up(a.sec, agents->secretion_rates, A * S);
+			up(a.sat, agents->saturation_densities, A * S);
+			up(a.upt, agents->uptake_rates, A * S);
+			up(a.nex, agents->net_export_rates, A * S);
+			up(a.intern, agents->internalized_substrates, A * S);
+			if (agents->fraction_released_at_death)
+				up(a.frac, agents->fraction_released_at_death, A * S);
+			else
+				BFVM_HIP(hipMemsetAsync(a.frac, 0, sizeof(float) * A * S, ctx->stream));
+			up(a.vol, agents->volumes, A);
+			up(a.pos, agents->positions, A * dims);
+		}
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_load_densities(bfvm_ctx* ctx, float* densities)
+{
+	return guarded("bfvm_load_densities", [&] {
+		require_init(ctx);
+		if (!densities)
+			throw std::invalid_argument("null densities");
+		BFVM_HIP(hipSetDevice(ctx->device));
+		BFVM_HIP(hipMemcpyAsync(densities, ctx->dens, sizeof(float) * ctx->mesh.V * ctx->mesh.S,
+								hipMemcpyDeviceToHost, ctx->stream));
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_load_gradients(bfvm_ctx* ctx, float* gradients)
+{
+	return guarded("bfvm_load_gradients", [&] {
+		require_init(ctx);
+		if (!gradients)
+			throw std::invalid_argument("null gradients");
+		BFVM_HIP(hipSetDevice(ctx->device));
+		BFVM_HIP(hipMemcpyAsync(gradients, ctx->grad, sizeof(float) * ctx->grad_elems, hipMemcpyDeviceToHost,
+								ctx->stream));
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_load_internalized(bfvm_ctx* ctx, float* internalized_substrates)
+{
+	return guarded("bfvm_load_internalized", [&] {
+		require_init(ctx);
+		BFVM_HIP(hipSetDevice(ctx->device));
+		const size_t n = (size_t)ctx->ag.count * ctx->mesh.S;
+		if (n > 0)
+		{
+			if (!internalized_substrates)
+				throw std::invalid_argument("null output");
+			BFVM_HIP(hipMemcpyAsync(internalized_substrates, ctx->ag.intern, sizeof(float) * n,
+									hipMemcpyDeviceToHost, ctx->stream));
+		}
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_diffusion_solve(bfvm_ctx* ctx)
+{
+	return guarded("bfvm_diffusion_solve", [&] {
+		require_init(ctx);
+		BFVM_HIP(hipSetDevice(ctx->device));
+		enqueue_diffusion(ctx);
+		return BFVM_OK;
+	});
+}
+
+int bfvm_dirichlet_solve(bfvm_ctx* ctx)
+{
+	return guarded("bfvm_dirichlet_solve", [&] {
+		require_init(ctx);
+		BFVM_HIP(hipSetDevice(ctx->device));
+		enqueue_dirichlet(ctx);
+		return BFVM_OK;
+	});
+}
+
+int bfvm_gradient_solve(bfvm_ctx* ctx)
+{
+	return guarded("bfvm_gradient_solve", [&] {
+		require_init(ctx);
+		check_gradient(ctx);
+		BFVM_HIP(hipSetDevice(ctx->device));
+		launch_gradient(ctx);
+		return BFVM_OK;
+	});
+}
+
+int bfvm_cell_simulate(bfvm_ctx* ctx, int32_t recompute)
+{
+	return guarded("bfvm_cell_simulate", [&] {
+		require_init(ctx);
+		BFVM_HIP(hipSetDevice(ctx->device));
+		launch_cell(ctx, recompute != 0);
+		return BFVM_OK;
+	});
+}
+
+int bfvm_cell_release(bfvm_ctx* ctx, int32_t agent_index)
+{
+	return guarded("bfvm_cell_release", [&] {
+		require_init(ctx);
+		if (!ctx->compute_internalized)
+			return BFVM_OK; // host/cell_solver.cpp:342-343
+		if (agent_index < 0 || agent_index >= ctx->ag.count)
+			throw std::invalid_argument("agent index out of range");
+		BFVM_HIP(hipSetDevice(ctx->device));
+		launch_cell_release(ctx, agent_index);
+		return BFVM_OK;
+	});
+}
+
+int bfvm_timestep(bfvm_ctx* ctx, int32_t recompute, int32_t with_gradient)
+{
+	return guarded("bfvm_timestep", [&] {
+		require_init(ctx);
+		if (with_gradient)
+			check_gradient(ctx);
+		BFVM_HIP(hipSetDevice(ctx->device));
+		auto& slot = ctx->graphs[(recompute ? 2 : 0) + (with_gradient ? 1 : 0)];
+		if (!slot.exec || slot.epoch != ctx->epoch)
+		{
+			if (slot.exec)
+				BFVM_HIP(hipGraphExecDestroy(slot.exec));
+			if (slot.graph)
+				BFVM_HIP(hipGraphDestroy(slot.graph));
+			slot = graph_slot {};
+			BFVM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+			try
+			{
+				launch_cell(ctx, recompute != 0);
+				enqueue_diffusion(ctx);
+				if (with_gradient)
+					launch_gradient(ctx);
+			}
+			catch (...)
+			{
+				hipGraph_t g = nullptr;
+				(void)hipStreamEndCapture(ctx->stream, &g);
+				if (g)
+					(void)hipGraphDestroy(g);
+				throw;
+			}
+			BFVM_HIP(hipStreamEndCapture(ctx->stream, &slot.graph));
+			BFVM_HIP(hipGraphInstantiate(&slot.exec, slot.graph, nullptr, nullptr, 0));
+			slot.epoch = ctx->epoch;
+		}
+		BFVM_HIP(hipGraphLaunch(slot.exec, ctx->stream));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_wait_for_all(bfvm_ctx* ctx)
+{
+	return guarded("bfvm_wait_for_all", [&] {
+		if (!ctx)
+			throw std::invalid_argument("null context");
+		BFVM_HIP(hipSetDevice(ctx->device));
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_get_device_buffers(bfvm_ctx* ctx, bfvm_device_buffers* out)
+{
+	return guarded("bfvm_get_device_buffers", [&] {
+		require_init(ctx);
+		if (!out)
+			throw std::invalid_argument("null output");
+		out->densities = ctx->dens;
+		out->gradients = ctx->grad;
+		out->internalized_substrates = ctx->ag.intern;
+		out->voxel_count = (int64_t)ctx->mesh.V;
+		for (int d = 0; d < 3; d++)
+			out->grid_shape[d] = ctx->mesh.grid[d];
+		out->substrates_count = ctx->mesh.S;
+		out->stream = (void*)ctx->stream;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_sweep_partition(bfvm_ctx* ctx, int32_t axis, int32_t* blocks, int32_t* block_length)
+{
+	return guarded("bfvm_sweep_partition", [&] {
+		require_init(ctx);
+		if (axis < 0 || axis >= ctx->mesh.dims)
+			throw std::invalid_argument("axis out of range");
+		const auto& p = ctx->axes[axis].part;
+		if (blocks)
+			*blocks = p.P;
+		if (block_length)
+			*block_length = p.P ? p.base + (p.rem > 0 ? 1 : 0) : 1;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_exact_coefficients(bfvm_ctx* ctx, int32_t axis, float* b, float* c, float* e)
+{
+	return guarded("bfvm_exact_coefficients", [&] {
+		require_init(ctx);
+		if (axis < 0 || axis >= ctx->mesh.dims)
+			throw std::invalid_argument("axis out of range");
+		BFVM_HIP(hipSetDevice(ctx->device));
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		const auto& t = ctx->axes[axis];
+		const size_t S = ctx->mesh.S;
+		if (b)
+			BFVM_HIP(hipMemcpy(b, t.exact_b, sizeof(float) * t.n * S, hipMemcpyDeviceToHost));
+		if (c && t.n > 1)
+			BFVM_HIP(hipMemcpy(c, t.exact_c, sizeof(float) * S, hipMemcpyDeviceToHost));
+		if (e && t.n > 1)
+			BFVM_HIP(hipMemcpy(e, t.exact_e, sizeof(float) * (t.n - 1) * S, hipMemcpyDeviceToHost));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_voxel_positions(bfvm_ctx* ctx, int32_t count, const float* positions, int32_t* out3)
+{
+	return guarded("bfvm_voxel_positions", [&] {
+		require_init(ctx);
+		if (count < 0 || (count > 0 && (!positions || !out3)))
+			throw std::invalid_argument("bad arguments");
+		if (count == 0)
+			return BFVM_OK;
+		BFVM_HIP(hipSetDevice(ctx->device));
+		float* d_pos = nullptr;
+		int* d_out = nullptr;
+		BFVM_HIP(hipMalloc(&d_pos, sizeof(float) * count * ctx->mesh.dims));
+		hipError_t err = hipMalloc(&d_out, sizeof(int) * count * 3);
+		if (err != hipSuccess)
+		{
+			(void)hipFree(d_pos);
+			BFVM_HIP(err);
+		}
+		try
+		{
+			BFVM_HIP(hipMemcpy(d_pos, positions, sizeof(float) * count * ctx->mesh.dims, hipMemcpyHostToDevice));
+			launch_voxel_positions(ctx, count, d_pos, d_out);
+			BFVM_HIP(hipStreamSynchronize(ctx->stream));
+			BFVM_HIP(hipMemcpy(out3, d_out, sizeof(int) * count * 3, hipMemcpyDeviceToHost));
+		}
+		catch (...)
+		{
+			(void)hipFree(d_pos);
+			(void)hipFree(d_out);
+			throw;
+		}
+		BFVM_HIP(hipFree(d_pos));
+		BFVM_HIP(hipFree(d_out));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_benchmark(bfvm_ctx* ctx, int32_t what, int32_t iterations, float* avg_ms)
+{
+	return guarded("bfvm_benchmark", [&] {
+		require_init(ctx);
+		if (iterations < 1 || !avg_ms)
+			throw std::invalid_argument("iterations must be positive and avg_ms non-null");
+		const int dims = ctx->mesh.dims;
+		if ((what == BFVM_KERNEL_SWEEP_Y && dims < 2) || (what == BFVM_KERNEL_SWEEP_Z && dims < 3))
+			throw std::invalid_argument("the mesh has no such axis");
+		if (what == BFVM_KERNEL_GRADIENT || what == BFVM_PHASE_GRADIENT || what == BFVM_PHASE_TIMESTEP)
+			check_gradient(ctx);
+		BFVM_HIP(hipSetDevice(ctx->device));
+		auto one = [&]() {
+			switch (what)
+			{
+			case BFVM_KERNEL_SWEEP_X:
+			case BFVM_KERNEL_SWEEP_Y:
+			case BFVM_KERNEL_SWEEP_Z:
+				launch_sweep(ctx, what - BFVM_KERNEL_SWEEP_X);
+				break;
+			case BFVM_KERNEL_GRADIENT:
+			case BFVM_PHASE_GRADIENT:
+				launch_gradient(ctx);
+				break;
+			case BFVM_KERNEL_DIRICHLET_INTERIOR:
+				launch_dirichlet_interior(ctx);
+				break;
+			case BFVM_KERNEL_DIRICHLET_FACES:
+				launch_dirichlet_faces(ctx);
+				break;
+			case BFVM_PHASE_SECRETION:
+				launch_cell(ctx, false);
+				break;
+			case BFVM_PHASE_SECRETION_RECOMPUTE:
+				launch_cell(ctx, true);
+				break;
+			case BFVM_PHASE_DIFFUSION:
+				enqueue_diffusion(ctx);
+				break;
+			case BFVM_PHASE_TIMESTEP:
+			{
+				int rc = bfvm_timestep(ctx, 0, 1);
+				if (rc != BFVM_OK)
+					throw std::runtime_error(g_error);
+				break;
+			}
+			default:
+				throw std::invalid_argument("unknown kernel or phase id");
+			}
+		};
+		// one untimed launch so a graph is captured and code objects are loaded
+		one();
+		hipEvent_t start = nullptr, stop = nullptr;
+		BFVM_HIP(hipEventCreate(&start));
+		hipError_t err = hipEventCreate(&stop);
+		if (err != hipSuccess)
+		{
+			(void)hipEventDestroy(start);
+			BFVM_HIP(err);
+		}
+		try
+		{
+			BFVM_HIP(hipEventRecord(start, ctx->stream));
+			for (int i = 0; i < iterations; i++)
+				one();
+			BFVM_HIP(hipEventRecord(stop, ctx->stream));
+			BFVM_HIP(hipEventSynchronize(stop));
+			float ms = 0;
+			BFVM_HIP(hipEventElapsedTime(&ms, start, stop));
+			*avg_ms = ms / iterations;
+		}
+		catch (...)
+		{
+			(void)hipEventDestroy(start);
+			(void)hipEventDestroy(stop);
+			throw;
+		}
+		BFVM_HIP(hipEventDestroy(start));
+		BFVM_HIP(hipEventDestroy(stop));
+		return BFVM_OK;
+	});
+}
+
+} // extern "C"

@@ -1,0 +1,277 @@
+// Agent secretion and uptake on gfx950 (host/cell_solver.cpp:15-346; formulas host/cell_solver.h:383-409).
+//
+//   num = S*T*dt*vol/vv,  den = (U+S)*dt*vol/vv,  fac = N*dt/vv          (per agent and substrate)
+//   D  <- (D + sum num) / (1 + sum den) + sum fac                        (per voxel)
+//   I  -= vv*((-den*D + num)/(1 + den) + fac)                            (per agent, old D)
+//
+// Same-voxel contributions are combined per recompute by a claim ("ballot") per voxel: the first agent
+// to compare-and-swap the voxel's ballot from -1 wins and every agent adds its terms into the winner's
+// reduced slot with float atomics (the winner adds den + 1).  A device flag records whether any voxel had
+// a second agent; it selects, on the device, between the fused per-agent update (no conflict) and the
+// two-phase update (internalized with the old D for every agent, then the winners update D) — the same
+// decision the reference makes on the host (host/cell_solver.cpp:208-244).
+//
+// Agents whose position maps outside the grid (or is not finite) are ignored; the reference would index
+// out of bounds (src/mesh.cpp:47-53 does not clamp).
+#include "bfvm_internal.hpp"
+
+namespace bfvm {
+
+struct cell_geom
+{
+	int dims;
+	int mins[3];
+	int vs[3];
+	int grid[3];
+	int S;
+};
+
+// mesh.cpp:47-53: (int)((p - (float)min) / (float)dx), IEEE division, truncation toward zero.
+// Returns the linear voxel ([z][y][x]) or -1 when outside.
+__device__ __forceinline__ int64_t agent_voxel(const cell_geom& g, const float* __restrict__ pos, int64_t i)
+{
+	int idx[3] = { 0, 0, 0 };
+	for (int d = 0; d < g.dims; d++)
+	{
+		const float q = (pos[i * g.dims + d] - (float)g.mins[d]) / (float)g.vs[d];
+		if (!(q > -1.0f) || !(q < (float)g.grid[d]))
+			return -1;
+		idx[d] = (int)q;
+	}
+	return ((int64_t)idx[2] * g.grid[1] + idx[1]) * g.grid[0] + idx[0];
+}
+
+// recompute, part 1 (host/cell_solver.cpp:35-85): intermediates, ballot reset, reduced-slot reset.
+__global__ __launch_bounds__(256) void cell_prepare_kernel(cell_geom g, int A, float voxel_volume, float dt,
+														   const float* __restrict__ pos, const float* __restrict__ vol,
+														   const float* __restrict__ sec, const float* __restrict__ sat,
+														   const float* __restrict__ upt, const float* __restrict__ nex,
+														   float* __restrict__ num, float* __restrict__ den,
+														   float* __restrict__ fac, float* __restrict__ rnum,
+														   float* __restrict__ rden, float* __restrict__ rfac,
+														   int* __restrict__ ballots, int* __restrict__ conflict)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i == 0)
+		atomicExch(conflict, 0);
+	if (i >= A)
+		return;
+	const int64_t v = agent_voxel(g, pos, i);
+	if (v >= 0)
+		ballots[v] = -1;
+	const float cv = vol[i];
+	for (int s = 0; s < g.S; s++)
+	{
+		const int64_t k = i * g.S + s;
+		num[k] = sec[k] * sat[k] * dt * cv / voxel_volume;
+		den[k] = (upt[k] + sec[k]) * dt * cv / voxel_volume;
+		fac[k] = nex[k] * dt / voxel_volume;
+		rnum[k] = 0;
+		rden[k] = 0;
+		rfac[k] = 0;
+	}
+}
+
+// recompute, part 2 (host/cell_solver.cpp:87-135): claim the voxel, add into the winner's slot.
+__global__ __launch_bounds__(256) void cell_ballot_kernel(cell_geom g, int A, const float* __restrict__ pos,
+														  const float* __restrict__ num, const float* __restrict__ den,
+														  const float* __restrict__ fac, float* __restrict__ rnum,
+														  float* __restrict__ rden, float* __restrict__ rfac,
+														  int* __restrict__ ballots, int* __restrict__ conflict)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= A)
+		return;
+	const int64_t v = agent_voxel(g, pos, i);
+	if (v < 0)
+		return;
+	const int prev = atomicCAS(&ballots[v], -1, (int)i);
+	if (prev == -1)
+	{
+		for (int s = 0; s < g.S; s++)
+		{
+			const int64_t k = i * g.S + s;
+			atomicAdd(&rnum[k], num[k]);
+			atomicAdd(&rden[k], den[k] + 1);
+			atomicAdd(&rfac[k], fac[k]);
+		}
+	}
+	else
+	{
+		atomicOr(conflict, 1);
+		for (int s = 0; s < g.S; s++)
+		{
+			const int64_t k = i * g.S + s;
+			const int64_t w = (int64_t)prev * g.S + s;
+			atomicAdd(&rnum[w], num[k]);
+			atomicAdd(&rden[w], den[k]);
+			atomicAdd(&rfac[w], fac[k]);
+		}
+	}
+}
+
+// Apply, part 1 (host/cell_solver.cpp:137-152, 173-195, 208-234).  One thread per (agent, substrate).
+// No conflict and internalized: the fused update of I and D.  Conflict and internalized: I with the
+// agent's own terms and the current D (D is updated by part 2 after the kernel boundary).
+__global__ __launch_bounds__(256) void cell_apply_kernel(cell_geom g, int A, int with_internalized, int voxel_volume,
+														 const float* __restrict__ pos, float* __restrict__ dens,
+														 float* __restrict__ intern, const float* __restrict__ num,
+														 const float* __restrict__ den, const float* __restrict__ fac,
+														 const float* __restrict__ rnum,
+														 const float* __restrict__ rden,
+														 const float* __restrict__ rfac,
+														 const int* __restrict__ conflict)
+{
+	if (!with_internalized)
+		return;
+	const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (int64_t)A * g.S)
+		return;
+	const int64_t i = idx / g.S;
+	const int s = (int)(idx % g.S);
+	const int64_t v = agent_voxel(g, pos, i);
+	if (v < 0)
+		return;
+	float* D = dens + v * g.S + s;
+	const float vv = (float)voxel_volume;
+	if (*conflict == 0)
+	{
+		const float d = *D;
+		const float rn = rnum[idx], rd = rden[idx], rf = rfac[idx];
+		intern[idx] = intern[idx] - vv * (((1 - rd) * d + rn) / (rd) + rf);
+		*D = (d + rn) / rd + rf;
+	}
+	else
+	{
+		const float d = *D;
+		intern[idx] = intern[idx] - vv * (((-den[idx]) * d + num[idx]) / (1 + den[idx]) + fac[idx]);
+	}
+}
+
+// Apply, part 2 (host/cell_solver.cpp:154-171, 236-244): the ballot winner of each voxel updates D with
+// the reduced sums.  Skipped when part 1 already did the fused update.
+__global__ __launch_bounds__(256) void cell_densities_kernel(cell_geom g, int A, int with_internalized,
+															 const float* __restrict__ pos, float* __restrict__ dens,
+															 const float* __restrict__ rnum,
+															 const float* __restrict__ rden,
+															 const float* __restrict__ rfac,
+															 const int* __restrict__ ballots,
+															 const int* __restrict__ conflict)
+{
+	if (with_internalized && *conflict == 0)
+		return;
+	const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (int64_t)A * g.S)
+		return;
+	const int64_t i = idx / g.S;
+	const int s = (int)(idx % g.S);
+	const int64_t v = agent_voxel(g, pos, i);
+	if (v < 0)
+		return;
+	if (ballots[v] != (int)i)
+		return;
+	float* D = dens + v * g.S + s;
+	*D = (*D + rnum[idx]) / rden[idx] + rfac[idx];
+}
+
+// host/cell_solver.cpp:301-328: D += I*F/vv, I = 0 for one agent.
+__global__ void cell_release_kernel(cell_geom g, int64_t i, float voxel_volume, const float* __restrict__ pos,
+									float* __restrict__ dens, float* __restrict__ intern,
+									const float* __restrict__ frac)
+{
+	const int s = threadIdx.x + blockIdx.x * blockDim.x;
+	if (s >= g.S)
+		return;
+	const int64_t v = agent_voxel(g, pos, i);
+	if (v < 0)
+		return;
+	const int64_t k = i * g.S + s;
+	dens[v * g.S + s] = dens[v * g.S + s] + intern[k] * frac[k] / voxel_volume;
+	intern[k] = 0;
+}
+
+__global__ void voxel_positions_kernel(cell_geom g, int count, const float* __restrict__ pos, int* __restrict__ out)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= count)
+		return;
+	for (int d = 0; d < 3; d++)
+	{
+		int r = 0;
+		if (d < g.dims)
+		{
+			const float q = (pos[i * g.dims + d] - (float)g.mins[d]) / (float)g.vs[d];
+			r = (q > -1.0f && q < (float)g.grid[d]) ? (int)q : -1;
+		}
+		out[i * 3 + d] = r;
+	}
+}
+
+static cell_geom make_geom(const bfvm_ctx* ctx)
+{
+	cell_geom g;
+	g.dims = ctx->mesh.dims;
+	for (int d = 0; d < 3; d++)
+	{
+		g.mins[d] = ctx->mesh.mins[d];
+		g.vs[d] = ctx->mesh.vs[d];
+		g.grid[d] = ctx->mesh.grid[d];
+	}
+	g.S = ctx->mesh.S;
+	return g;
+}
+
+static int voxel_volume(const bfvm_ctx* ctx) { return ctx->mesh.vs[0] * ctx->mesh.vs[1] * ctx->mesh.vs[2]; }
+
+void launch_cell(bfvm_ctx* ctx, bool recompute)
+{
+	const auto& a = ctx->ag;
+	const int A = a.count;
+	const cell_geom g = make_geom(ctx);
+	const unsigned agent_blocks = (unsigned)((A + 255) / 256);
+	const int64_t pairs = (int64_t)A * g.S;
+	const unsigned pair_blocks = (unsigned)((pairs + 255) / 256);
+	if (recompute)
+	{
+		// at least one block so the conflict flag is cleared even without agents
+		hipLaunchKernelGGL(cell_prepare_kernel, dim3(agent_blocks > 0 ? agent_blocks : 1), dim3(256), 0, ctx->stream,
+						   g, A, (float)voxel_volume(ctx), ctx->mesh.dt, a.pos, a.vol, a.sec, a.sat, a.upt, a.nex,
+						   a.num, a.den, a.fac, a.rnum, a.rden, a.rfac, ctx->ballots, ctx->conflict);
+		BFVM_HIP(hipGetLastError());
+		if (A > 0)
+		{
+			hipLaunchKernelGGL(cell_ballot_kernel, dim3(agent_blocks), dim3(256), 0, ctx->stream, g, A, a.pos, a.num,
+							   a.den, a.fac, a.rnum, a.rden, a.rfac, ctx->ballots, ctx->conflict);
+			BFVM_HIP(hipGetLastError());
+		}
+	}
+	if (A == 0)
+		return;
+	hipLaunchKernelGGL(cell_apply_kernel, dim3(pair_blocks), dim3(256), 0, ctx->stream, g, A,
+					   (int)ctx->compute_internalized, voxel_volume(ctx), a.pos, ctx->dens, a.intern, a.num, a.den,
+					   a.fac, a.rnum, a.rden, a.rfac, ctx->conflict);
+	BFVM_HIP(hipGetLastError());
+	hipLaunchKernelGGL(cell_densities_kernel, dim3(pair_blocks), dim3(256), 0, ctx->stream, g, A,
+					   (int)ctx->compute_internalized, a.pos, ctx->dens, a.rnum, a.rden, a.rfac, ctx->ballots,
+					   ctx->conflict);
+	BFVM_HIP(hipGetLastError());
+}
+
+void launch_cell_release(bfvm_ctx* ctx, int index)
+{
+	const auto& a = ctx->ag;
+	const cell_geom g = make_geom(ctx);
+	hipLaunchKernelGGL(cell_release_kernel, dim3((g.S + 63) / 64), dim3(64), 0, ctx->stream, g, (int64_t)index,
+					   (float)voxel_volume(ctx), a.pos, ctx->dens, a.intern, a.frac);
+	BFVM_HIP(hipGetLastError());
+}
+
+void launch_voxel_positions(bfvm_ctx* ctx, int count, const float* d_pos, int* d_out)
+{
+	const cell_geom g = make_geom(ctx);
+	hipLaunchKernelGGL(voxel_positions_kernel, dim3((count + 255) / 256), dim3(256), 0, ctx->stream, g, count, d_pos,
+					   d_out);
+	BFVM_HIP(hipGetLastError());
+}
+
+} // namespace bfvm

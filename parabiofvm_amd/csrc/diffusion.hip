@@ -1,0 +1,781 @@
+// LOD implicit diffusion on gfx950: Thomas coefficient setup and the x/y/z sweeps.
+//
+// The system of one line (host/diffusion_solver.h:359-390, coefficients host/diffusion_solver.cpp:72-142):
+//   a x_{i-1} + b_i x_i + c x_{i+1} = d_i,  a = c = -dt*D/dx^2,
+//   b_end = 1 + decay*dt/dims + dt*D/dx^2,  b_mid = 1 + decay*dt/dims + 2*dt*D/dx^2,
+// and for a line of one voxel b = 1 + decay*dt/dims (host/diffusion_solver.cpp:76-85).
+//
+// Two sweep algorithms:
+//  * exact: one thread per line, the reference's sequential recurrence with its FP32 coefficients in its
+//    operation order (host/diffusion_solver.cpp:297-334, 378-480); bit-identical to the host path.
+//  * partitioned: the line is split into P blocks held in registers, one thread per (line, block).  Each
+//    block is reduced to its two end unknowns by the modified Thomas algorithm (the PaScaL_TDMA scheme the
+//    reference precomputes in host/diffusion_solver.cpp:144-295 for its 2-D device kernels), the 2P-unknown
+//    reduced system is solved per line from LDS, and the block interiors are reconstructed.  Every element
+//    is read once and written once.  Coefficients are derived in FP64 on the GPU and rounded to FP32.
+//
+// Both apply the Dirichlet boundary faces in their store epilogue (the Dirichlet pass that follows each
+// sweep in host/diffusion_solver.cpp:529,538,543); the interior list is applied by a separate kernel.
+#include <algorithm>
+#include <stdexcept>
+
+#include "bfvm_internal.hpp"
+
+namespace bfvm {
+
+// ------------------------------------------------------------------------------------------------
+// coefficients
+// ------------------------------------------------------------------------------------------------
+
+// Reference-order FP32 coefficients of one axis: host/diffusion_solver.cpp:72-142 (copies = 1).
+// One thread per substrate; the recurrence over i is sequential.
+__global__ void exact_coefficients_kernel(float* __restrict__ b, float* __restrict__ c, float* __restrict__ e,
+										  const float* __restrict__ D, const float* __restrict__ decay, float dt,
+										  int shape, int dims, int n, int S)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= S)
+		return;
+	if (n == 1)
+	{
+		b[s] = 1 / (1 + decay[s] * dt / dims);
+		return;
+	}
+	const float cs = -dt * D[s] / (shape * shape);
+	c[s] = cs;
+	const float b_end = 1 + decay[s] * dt / dims + dt * D[s] / (shape * shape);
+	const float b_mid = 1 + decay[s] * dt / dims + 2 * dt * D[s] / (shape * shape);
+	float prev = 1 / b_end;
+	b[s] = prev;
+	for (int i = 1; i < n; i++)
+	{
+		const float bi = (i == n - 1) ? b_end : b_mid;
+		const float cur = 1 / (bi - cs * cs * prev);
+		b[(size_t)i * S + s] = cur;
+		e[(size_t)(i - 1) * S + s] = cs * prev;
+		prev = cur;
+	}
+}
+
+// Partitioned (modified Thomas) coefficients of one axis in FP64, rounded to FP32.  One thread per substrate.
+// Block of m elements, local j, diagonal B_j (B_end at the line's ends, B_mid elsewhere), off-diagonal A:
+//   forward  j<2: r=1/B_j, alpha_j=A r, gamma_j=A r;  j>=2: r=1/(B_j - A gamma_{j-1}),
+//            alpha_j=-A alpha_{j-1} r, gamma_j=A r       -> row j: alpha_j x_0 + x_j + gamma_j x_{j+1} = delta_j
+//   backward j=m-3..1: alpha'_j = alpha_j - gamma_j alpha'_{j+1}, gamma'_j = -gamma_j gamma'_{j+1}
+//            j=0 (m>=3): rho = 1/(1 - gamma_0 alpha'_1), alpha'_0 = rho alpha_0, gamma'_0 = -rho gamma_0 gamma'_1
+//                                                         -> row j: alpha'_j x_0 + x_j + gamma'_j x_{m-1} = delta'_j
+// The reduced system of the line has unknowns u_{2k} = first, u_{2k+1} = last of block k, with rows
+//   u_{2k}:   alpha'_0 u_{2k-1} + u_{2k} + gamma'_0 u_{2k+1}
+//   u_{2k+1}: alpha_{m-1} u_{2k} + u_{2k+1} + gamma_{m-1} u_{2k+2}
+// and is solved by the Thomas recurrence with unit diagonal; its coefficients L_q, inv_q, c'_q are stored.
+// Table layout: coef[f][type][j][s], f = 0 r, 1 gamma, 2 alpha' (rho in the j=0 slot), 3 gamma';
+// red[f][q][s], f = 0 L, 1 inv, 2 c'.
+constexpr int max_types = 4;
+
+__global__ void partition_coefficients_kernel(float* __restrict__ coef, float* __restrict__ red,
+											  float* __restrict__ A_out, const float* __restrict__ D,
+											  const float* __restrict__ decay, float dt, int shape, int dims, int n,
+											  int S, int P, int base, int rem, int ntypes, int mb)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= S)
+		return;
+	const double ddt = dt, dD = D[s], dl = decay[s];
+	const double h2 = (double)shape * (double)shape;
+	const double A = -ddt * dD / h2;
+	const double B_end = 1.0 + dl * ddt / dims + ddt * dD / h2;
+	const double B_mid = 1.0 + dl * ddt / dims + 2.0 * ddt * dD / h2;
+	A_out[s] = (float)A;
+
+	double ap0[max_types], gp0[max_types], al_last[max_types], ga_last[max_types];
+	const size_t fstride = (size_t)ntypes * mb * S;
+
+	for (int t = 0; t < ntypes; t++)
+	{
+		int m;
+		bool first, last;
+		if (P == 1)
+		{
+			m = n;
+			first = last = true;
+		}
+		else
+		{
+			first = t == 0;
+			last = t == 3;
+			m = (t == 0) ? base + (rem > 0 ? 1 : 0) : (t == 1 ? base + 1 : base);
+		}
+		// forward: alpha, gamma kept in doubles for the backward pass
+		double alpha[129], gamma[129];
+		for (int j = 0; j < m; j++)
+		{
+			const bool end = (first && j == 0) || (last && j == m - 1);
+			const double Bj = end ? B_end : B_mid;
+			double r;
+			if (j < 2)
+			{
+				r = 1.0 / Bj;
+				alpha[j] = A * r;
+			}
+			else
+			{
+				r = 1.0 / (Bj - A * gamma[j - 1]);
+				alpha[j] = -A * alpha[j - 1] * r;
+			}
+			gamma[j] = A * r;
+			coef[0 * fstride + ((size_t)t * mb + j) * S + s] = (float)r;
+			coef[1 * fstride + ((size_t)t * mb + j) * S + s] = (float)gamma[j];
+		}
+		// backward
+		double apn = 0, gpn = 0; // alpha', gamma' of row j+1
+		for (int j = m - 2; j >= 1; j--)
+		{
+			double apj, gpj;
+			if (j == m - 2)
+			{
+				apj = alpha[j];
+				gpj = gamma[j];
+			}
+			else
+			{
+				apj = alpha[j] - gamma[j] * apn;
+				gpj = -gamma[j] * gpn;
+			}
+			coef[2 * fstride + ((size_t)t * mb + j) * S + s] = (float)apj;
+			coef[3 * fstride + ((size_t)t * mb + j) * S + s] = (float)gpj;
+			apn = apj;
+			gpn = gpj;
+		}
+		if (m >= 3)
+		{
+			const double rho = 1.0 / (1.0 - gamma[0] * apn);
+			ap0[t] = rho * alpha[0];
+			gp0[t] = -rho * gamma[0] * gpn;
+			coef[2 * fstride + ((size_t)t * mb + 0) * S + s] = (float)rho;
+		}
+		else
+		{
+			ap0[t] = alpha[0];
+			gp0[t] = gamma[0];
+			coef[2 * fstride + ((size_t)t * mb + 0) * S + s] = 1.0f;
+		}
+		coef[3 * fstride + ((size_t)t * mb + 0) * S + s] = 0.0f;
+		al_last[t] = alpha[m - 1];
+		ga_last[t] = gamma[m - 1];
+	}
+
+	// reduced system, Thomas with unit diagonal
+	const size_t rstride = (size_t)2 * P * S;
+	double cprev = 0;
+	for (int q = 0; q < 2 * P; q++)
+	{
+		const int k = q / 2;
+		const int t = (P == 1) ? 0 : (k == 0 ? 0 : (k == P - 1 ? 3 : (k < rem ? 1 : 2)));
+		double L, U;
+		if ((q & 1) == 0)
+		{
+			L = (k > 0) ? ap0[t] : 0.0;
+			U = gp0[t];
+		}
+		else
+		{
+			L = al_last[t];
+			U = (k < P - 1) ? ga_last[t] : 0.0;
+		}
+		double inv, cp;
+		if (q == 0)
+		{
+			inv = 1.0;
+			cp = U;
+		}
+		else
+		{
+			inv = 1.0 / (1.0 - L * cprev);
+			cp = U * inv;
+		}
+		red[0 * rstride + (size_t)q * S + s] = (float)L;
+		red[1 * rstride + (size_t)q * S + s] = (float)inv;
+		red[2 * rstride + (size_t)q * S + s] = (float)cp;
+		cprev = cp;
+	}
+}
+
+// ------------------------------------------------------------------------------------------------
+// Dirichlet faces in a sweep's store epilogue
+// ------------------------------------------------------------------------------------------------
+
+// Per-thread view of the six faces for one substrate and a line along `axis` with the other two
+// coordinates fixed: the override from faces of axes before `axis` (lo), after it (hi), and the two faces
+// of `axis` itself.  Face precedence is x-, x+, y-, y+, z-, z+ (host/dirichlet_solver.cpp:91-110).
+struct face_epilogue
+{
+	bool lo_set = false, hi_set = false, cmin = false, cmax = false;
+	float lo_val = 0, hi_val = 0, vmin = 0, vmax = 0;
+
+	__device__ void init(const float* __restrict__ fval, const uint8_t* __restrict__ fcond, int S, int s, int axis,
+						 const int* coord, const int* grid, int dims)
+	{
+		if (!fval)
+			return;
+		for (int d = 0; d < dims; d++)
+		{
+			for (int mx = 0; mx < 2; mx++)
+			{
+				const int f = 2 * d + mx;
+				const bool cond = fcond[f * S + s] != 0;
+				const float val = fval[f * S + s];
+				if (d == axis)
+				{
+					if (mx == 0)
+					{
+						cmin = cond;
+						vmin = val;
+					}
+					else
+					{
+						cmax = cond;
+						vmax = val;
+					}
+					continue;
+				}
+				const bool on = mx == 0 ? coord[d] == 0 : coord[d] == grid[d] - 1;
+				if (on && cond)
+				{
+					if (d < axis)
+					{
+						lo_set = true;
+						lo_val = val;
+					}
+					else
+					{
+						hi_set = true;
+						hi_val = val;
+					}
+				}
+			}
+		}
+	}
+
+	__device__ __forceinline__ float apply(float v, int i, int n) const
+	{
+		if (lo_set)
+			v = lo_val;
+		if (i == 0 && cmin)
+			v = vmin;
+		if (i == n - 1 && cmax)
+			v = vmax;
+		if (hi_set)
+			v = hi_val;
+		return v;
+	}
+};
+
+struct sweep_geometry
+{
+	float* data;
+	int X, Y, Z, S, dims;
+	const float* fval;
+	const uint8_t* fcond;
+};
+
+// Line geometry per axis: the element (line, i) lives at base + i*stride.
+//  x: line = (row r = z*Y + y, s), stride S
+//  y: line = (z, q) with q = x*S + s, stride X*S
+//  z: line = q in [0, Y*X*S), stride Y*X*S
+template <int AXIS>
+__device__ __forceinline__ void line_coords(const sweep_geometry& g, int64_t line, int64_t& base, int& s, int* coord)
+{
+	const int64_t XS = (int64_t)g.X * g.S;
+	if constexpr (AXIS == 0)
+	{
+		const int64_t row = line / g.S;
+		s = (int)(line % g.S);
+		base = row * XS + s;
+		coord[0] = 0;
+		coord[1] = (int)(row % g.Y);
+		coord[2] = (int)(row / g.Y);
+	}
+	else if constexpr (AXIS == 1)
+	{
+		const int64_t z = line / XS;
+		const int64_t q = line % XS;
+		s = (int)(q % g.S);
+		base = z * (int64_t)g.Y * XS + q;
+		coord[0] = (int)(q / g.S);
+		coord[1] = 0;
+		coord[2] = (int)z;
+	}
+	else
+	{
+		s = (int)(line % g.S);
+		base = line;
+		coord[0] = (int)((line / g.S) % g.X);
+		coord[1] = (int)(line / XS);
+		coord[2] = 0;
+	}
+}
+
+template <int AXIS>
+__device__ __forceinline__ int64_t line_stride(const sweep_geometry& g)
+{
+	if constexpr (AXIS == 0)
+		return g.S;
+	else if constexpr (AXIS == 1)
+		return (int64_t)g.X * g.S;
+	else
+		return (int64_t)g.Y * g.X * g.S;
+}
+
+// ------------------------------------------------------------------------------------------------
+// exact sweep: one thread per line (host/diffusion_solver.cpp:297-334 along x, :378-480 along y/z)
+// ------------------------------------------------------------------------------------------------
+
+template <int AXIS>
+__global__ __launch_bounds__(256) void sweep_exact_kernel(sweep_geometry g, int64_t lines, int n,
+														  const float* __restrict__ b, const float* __restrict__ c,
+														  const float* __restrict__ e)
+{
+	const int64_t line = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (line >= lines)
+		return;
+	int64_t base;
+	int s;
+	int coord[3];
+	line_coords<AXIS>(g, line, base, s, coord);
+	const int64_t stride = line_stride<AXIS>(g);
+	float* p = g.data + base;
+	const int grid[3] = { g.X, g.Y, g.Z };
+	face_epilogue fe;
+	fe.init(g.fval, g.fcond, g.S, s, AXIS, coord, grid, g.dims);
+
+	float prev = p[0];
+	for (int i = 1; i < n; i++)
+	{
+		float cur = p[i * stride];
+		cur = cur - e[(size_t)(i - 1) * g.S + s] * prev;
+		p[i * stride] = cur;
+		prev = cur;
+	}
+	const float cs = c[s];
+	float next = prev * b[(size_t)(n - 1) * g.S + s];
+	p[(int64_t)(n - 1) * stride] = fe.apply(next, n - 1, n);
+	for (int i = n - 2; i >= 0; i--)
+	{
+		float cur = p[i * stride];
+		cur = (cur - cs * next) * b[(size_t)i * g.S + s];
+		p[i * stride] = fe.apply(cur, i, n);
+		next = cur;
+	}
+}
+
+// Axis of length one: the sweep is x *= b (b = 1/(1 + decay*dt/dims)), then the faces.
+__global__ __launch_bounds__(256) void sweep_single_kernel(sweep_geometry g, int axis, const float* __restrict__ b)
+{
+	const int64_t total = (int64_t)g.X * g.Y * g.Z * g.S;
+	const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= total)
+		return;
+	const int s = (int)(idx % g.S);
+	const int64_t v = idx / g.S;
+	int coord[3] = { (int)(v % g.X), (int)((v / g.X) % g.Y), (int)(v / ((int64_t)g.X * g.Y)) };
+	const int grid[3] = { g.X, g.Y, g.Z };
+	face_epilogue fe;
+	fe.init(g.fval, g.fcond, g.S, s, axis, coord, grid, g.dims);
+	g.data[idx] = fe.apply(g.data[idx] * b[s], 0, 1);
+}
+
+// ------------------------------------------------------------------------------------------------
+// partitioned sweep
+// ------------------------------------------------------------------------------------------------
+
+struct part_args
+{
+	sweep_geometry g;
+	int n, P, base, rem, ntypes, mb;
+	const float* coef;
+	const float* red;
+	const float* A;
+	int64_t lines;		 // number of lines
+	int lines_per_group; // lines handled by one workgroup iteration
+	int64_t groups;
+};
+
+// Thread t of a workgroup handles block k of line `line_local`:
+//  x: t = (line_local / S) * (S*P) + k*S + s  -> lanes of one row read S consecutive floats per block
+//  y, z: t = k * lines_per_group + line_local -> lanes read consecutive floats of one plane row
+template <int AXIS>
+__device__ __forceinline__ void thread_slot(int t, int S, int P, int lpg, int& line_local, int& k)
+{
+	if constexpr (AXIS == 0)
+	{
+		const int s = t % S;
+		k = (t / S) % P;
+		line_local = (t / (S * P)) * S + s;
+		if (t >= lpg * P)
+			k = -1;
+	}
+	else
+	{
+		line_local = t % lpg;
+		k = t / lpg;
+		if (k >= P)
+			k = -1;
+	}
+}
+
+template <int M, int AXIS>
+__global__ __launch_bounds__(1024) void sweep_partitioned_kernel(part_args a)
+{
+	extern __shared__ float lds[];
+	const sweep_geometry& g = a.g;
+	const int S = g.S;
+	const int P = a.P;
+	const int lpg = a.lines_per_group;
+	const int ncoef = 4 * a.ntypes * a.mb * S;
+	const int nred = 3 * 2 * P * S;
+	float* lcoef = lds;
+	float* lred = lds + ncoef;
+	float* rv = lred + nred; // [2P][lpg]
+
+	for (int i = threadIdx.x; i < ncoef; i += blockDim.x)
+		lcoef[i] = a.coef[i];
+	for (int i = threadIdx.x; i < nred; i += blockDim.x)
+		lred[i] = a.red[i];
+
+	int line_local, k;
+	thread_slot<AXIS>(threadIdx.x, S, P, lpg, line_local, k);
+	const int64_t stride = line_stride<AXIS>(g);
+	const int grid[3] = { g.X, g.Y, g.Z };
+	const size_t fstride = (size_t)a.ntypes * a.mb * S;
+	const size_t rstride = (size_t)2 * P * S;
+
+	int start = 0, mk = 0, type = 0;
+	if (k >= 0)
+	{
+		start = k * a.base + min(k, a.rem);
+		mk = a.base + (k < a.rem ? 1 : 0);
+		type = (P == 1) ? 0 : (k == 0 ? 0 : (k == P - 1 ? 3 : (k < a.rem ? 1 : 2)));
+	}
+	__syncthreads();
+
+	for (int64_t group = blockIdx.x; group < a.groups; group += gridDim.x)
+	{
+		const int64_t line = group * lpg + line_local;
+		const bool active = k >= 0 && line < a.lines;
+		int64_t base = 0;
+		int s = 0;
+		int coord[3] = { 0, 0, 0 };
+		if (active)
+			line_coords<AXIS>(g, line, base, s, coord);
+		float* p = g.data + base + (int64_t)start * stride;
+		const float* cr = lcoef + 0 * fstride + (size_t)type * a.mb * S + s;
+		const float* cg = lcoef + 1 * fstride + (size_t)type * a.mb * S + s;
+		const float* ca = lcoef + 2 * fstride + (size_t)type * a.mb * S + s;
+		const float* cgp = lcoef + 3 * fstride + (size_t)type * a.mb * S + s;
+
+		float v[M];
+		if (active)
+		{
+#pragma unroll
+			for (int j = 0; j < M; j++)
+				if (j < mk)
+					v[j] = p[j * stride];
+			// forward elimination inside the block
+			const float As = a.A[s];
+#pragma unroll
+			for (int j = 0; j < M; j++)
+			{
+				if (j < mk)
+				{
+					const float r = cr[j * S];
+					if (j < 2)
+						v[j] = r * v[j];
+					else
+						v[j] = r * (v[j] - As * v[j - 1]);
+				}
+			}
+			// backward elimination towards the block's last unknown
+#pragma unroll
+			for (int j = M - 3; j >= 1; j--)
+				if (j <= mk - 3)
+					v[j] = v[j] - cg[j * S] * v[j + 1];
+			if (mk >= 3)
+				v[0] = ca[0] * (v[0] - cg[0] * v[1]);
+			rv[(2 * k) * lpg + line_local] = v[0];
+#pragma unroll
+			for (int j = 1; j < M; j++)
+				if (j == mk - 1)
+					rv[(2 * k + 1) * lpg + line_local] = v[j];
+			if (mk == 1)
+				rv[(2 * k + 1) * lpg + line_local] = v[0];
+		}
+		__syncthreads();
+		// reduced system of the line, one thread per line
+		if (active && k == 0)
+		{
+			const float* L = lred + 0 * rstride + s;
+			const float* inv = lred + 1 * rstride + s;
+			const float* cp = lred + 2 * rstride + s;
+			float* r = rv + line_local;
+			float Rp = r[0];
+			for (int q = 1; q < 2 * P; q++)
+			{
+				Rp = (r[q * lpg] - L[q * S] * Rp) * inv[q * S];
+				r[q * lpg] = Rp;
+			}
+			float u = Rp;
+			for (int q = 2 * P - 2; q >= 0; q--)
+			{
+				u = r[q * lpg] - cp[q * S] * u;
+				r[q * lpg] = u;
+			}
+		}
+		__syncthreads();
+		if (active)
+		{
+			const float u0 = rv[(2 * k) * lpg + line_local];
+			const float u1 = rv[(2 * k + 1) * lpg + line_local];
+			face_epilogue fe;
+			fe.init(g.fval, g.fcond, S, s, AXIS, coord, grid, g.dims);
+#pragma unroll
+			for (int j = 0; j < M; j++)
+			{
+				if (j < mk)
+				{
+					float x;
+					if (j == 0)
+						x = u0;
+					else if (j == mk - 1)
+						x = u1;
+					else
+						x = v[j] - ca[j * S] * u0 - cgp[j * S] * u1;
+					p[j * stride] = fe.apply(x, start + j, a.n);
+				}
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// ------------------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------------------
+
+static constexpr int chunk_sizes[] = { 8, 16, 32, 48, 64 };
+
+int supported_chunk(int request)
+{
+	for (int c : chunk_sizes)
+		if (request <= c)
+			return c;
+	return 64;
+}
+
+static sweep_geometry geometry(bfvm_ctx* ctx)
+{
+	sweep_geometry g;
+	g.data = ctx->dens;
+	g.X = ctx->mesh.grid[0];
+	g.Y = ctx->mesh.grid[1];
+	g.Z = ctx->mesh.grid[2];
+	g.S = ctx->mesh.S;
+	g.dims = ctx->mesh.dims;
+	g.fval = ctx->has_faces ? ctx->face_val : nullptr;
+	g.fcond = ctx->has_faces ? ctx->face_cond : nullptr;
+	return g;
+}
+
+void launch_exact_coefficients(bfvm_ctx* ctx, int axis)
+{
+	auto& t = ctx->axes[axis];
+	const int S = ctx->mesh.S;
+	hipLaunchKernelGGL(exact_coefficients_kernel, dim3((S + 63) / 64), dim3(64), 0, ctx->stream, t.exact_b, t.exact_c,
+					   t.exact_e, ctx->d_D, ctx->d_decay, ctx->mesh.dt, ctx->mesh.vs[axis], ctx->mesh.dims,
+					   ctx->mesh.grid[axis], S);
+	BFVM_HIP(hipGetLastError());
+}
+
+static void free_partition_tables(axis_tables& t)
+{
+	if (t.coef)
+		BFVM_HIP(hipFree(t.coef));
+	if (t.red)
+		BFVM_HIP(hipFree(t.red));
+	if (t.A)
+		BFVM_HIP(hipFree(t.A));
+	t.coef = t.red = t.A = nullptr;
+}
+
+// lines handled by one workgroup and its thread count for a partition of P blocks
+template <int AXIS>
+static void group_shape(int S, int P, int& lpg, int& threads)
+{
+	if (AXIS == 0)
+	{
+		const int per_row = S * P;
+		const int rows = std::max(1, 256 / per_row);
+		lpg = rows * S;
+		threads = ((rows * per_row + 63) / 64) * 64;
+	}
+	else
+	{
+		lpg = (64 * P <= 1024) ? 64 : 32;
+		threads = lpg * P;
+	}
+}
+
+static size_t partition_lds_bytes(const partition& p, int S, int lpg)
+{
+	return sizeof(float) * ((size_t)4 * p.ntypes * p.mb * S + (size_t)6 * p.P * S + (size_t)2 * p.P * lpg);
+}
+
+void build_partition(bfvm_ctx* ctx, int axis, int chunk)
+{
+	auto& t = ctx->axes[axis];
+	const int n = ctx->mesh.grid[axis];
+	const int S = ctx->mesh.S;
+	free_partition_tables(t);
+	t.part = partition {};
+	t.part.n = n;
+	if (n <= 1)
+		return;
+	int M = supported_chunk(chunk > 0 ? chunk : 32);
+	for (;; M = supported_chunk(M + 1))
+	{
+		partition p;
+		p.n = n;
+		p.chunk = M;
+		p.P = (n + M - 1) / M;
+		p.base = n / p.P;
+		p.rem = n % p.P;
+		p.ntypes = p.P == 1 ? 1 : 4;
+		p.mb = p.P == 1 ? n : p.base + 1;
+		int lpg, threads;
+		if (axis == 0)
+			group_shape<0>(S, p.P, lpg, threads);
+		else
+			group_shape<1>(S, p.P, lpg, threads);
+		const bool fits = threads <= 1024 && partition_lds_bytes(p, S, lpg) <= 64 * 1024 && (p.P == 1 || p.base >= 3);
+		if (fits || M == 64)
+		{
+			if (!fits)
+				throw std::runtime_error("partitioned sweep: axis too long for the register/LDS budget "
+										 "(use the exact sweep)");
+			t.part = p;
+			break;
+		}
+	}
+	const partition& p = t.part;
+	BFVM_HIP(hipMalloc(&t.coef, sizeof(float) * 4 * p.ntypes * p.mb * S));
+	BFVM_HIP(hipMalloc(&t.red, sizeof(float) * 3 * 2 * p.P * S));
+	BFVM_HIP(hipMalloc(&t.A, sizeof(float) * S));
+}
+
+void launch_partition_coefficients(bfvm_ctx* ctx, int axis)
+{
+	auto& t = ctx->axes[axis];
+	const partition& p = t.part;
+	if (p.P == 0)
+		return;
+	const int S = ctx->mesh.S;
+	hipLaunchKernelGGL(partition_coefficients_kernel, dim3((S + 63) / 64), dim3(64), 0, ctx->stream, t.coef, t.red,
+					   t.A, ctx->d_D, ctx->d_decay, ctx->mesh.dt, ctx->mesh.vs[axis], ctx->mesh.dims, p.n, S, p.P,
+					   p.base, p.rem, p.ntypes, p.mb);
+	BFVM_HIP(hipGetLastError());
+}
+
+template <int M, int AXIS>
+static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const axis_tables& t)
+{
+	const partition& p = t.part;
+	part_args a;
+	a.g = g;
+	a.n = p.n;
+	a.P = p.P;
+	a.base = p.base;
+	a.rem = p.rem;
+	a.ntypes = p.ntypes;
+	a.mb = p.mb;
+	a.coef = t.coef;
+	a.red = t.red;
+	a.A = t.A;
+	const int64_t total = (int64_t)g.X * g.Y * g.Z * g.S;
+	a.lines = total / p.n;
+	int lpg, threads;
+	group_shape<AXIS>(g.S, p.P, lpg, threads);
+	a.lines_per_group = lpg;
+	a.groups = (a.lines + lpg - 1) / lpg;
+	const size_t lds = partition_lds_bytes(p, g.S, lpg);
+	const int64_t max_blocks = 8192;
+	const int blocks = (int)std::min<int64_t>(a.groups, max_blocks);
+	hipLaunchKernelGGL((sweep_partitioned_kernel<M, AXIS>), dim3(blocks), dim3(threads), lds, ctx->stream, a);
+	BFVM_HIP(hipGetLastError());
+}
+
+template <int AXIS>
+static void launch_partitioned_axis(bfvm_ctx* ctx, const sweep_geometry& g, const axis_tables& t)
+{
+	switch (t.part.chunk)
+	{
+	case 8:
+		launch_partitioned_t<8, AXIS>(ctx, g, t);
+		break;
+	case 16:
+		launch_partitioned_t<16, AXIS>(ctx, g, t);
+		break;
+	case 32:
+		launch_partitioned_t<32, AXIS>(ctx, g, t);
+		break;
+	case 48:
+		launch_partitioned_t<48, AXIS>(ctx, g, t);
+		break;
+	default:
+		launch_partitioned_t<64, AXIS>(ctx, g, t);
+		break;
+	}
+}
+
+template <int AXIS>
+static void launch_exact_axis(bfvm_ctx* ctx, const sweep_geometry& g, const axis_tables& t)
+{
+	const int64_t total = (int64_t)g.X * g.Y * g.Z * g.S;
+	const int n = ctx->mesh.grid[AXIS];
+	const int64_t lines = total / n;
+	const int64_t blocks = (lines + 255) / 256;
+	hipLaunchKernelGGL((sweep_exact_kernel<AXIS>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream, g, lines, n,
+					   t.exact_b, t.exact_c, t.exact_e);
+	BFVM_HIP(hipGetLastError());
+}
+
+void launch_sweep(bfvm_ctx* ctx, int axis)
+{
+	const sweep_geometry g = geometry(ctx);
+	const axis_tables& t = ctx->axes[axis];
+	const int n = ctx->mesh.grid[axis];
+	if (n == 1)
+	{
+		const int64_t total = (int64_t)g.X * g.Y * g.Z * g.S;
+		hipLaunchKernelGGL(sweep_single_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, g,
+						   axis, t.exact_b);
+		BFVM_HIP(hipGetLastError());
+		return;
+	}
+	if (ctx->algorithm == BFVM_SWEEP_EXACT)
+	{
+		if (axis == 0)
+			launch_exact_axis<0>(ctx, g, t);
+		else if (axis == 1)
+			launch_exact_axis<1>(ctx, g, t);
+		else
+			launch_exact_axis<2>(ctx, g, t);
+		return;
+	}
+	if (axis == 0)
+		launch_partitioned_axis<0>(ctx, g, t);
+	else if (axis == 1)
+		launch_partitioned_axis<1>(ctx, g, t);
+	else
+		launch_partitioned_axis<2>(ctx, g, t);
+}
+
+} // namespace bfvm

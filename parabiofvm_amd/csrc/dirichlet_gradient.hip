@@ -1,0 +1,196 @@
+// Dirichlet conditions and central-difference gradients on gfx950.
+#include "bfvm_internal.hpp"
+
+namespace bfvm {
+
+// ------------------------------------------------------------------------------------------------
+// Dirichlet — host/dirichlet_solver.cpp:22-144
+// ------------------------------------------------------------------------------------------------
+
+// Boundary faces.  One thread per (face voxel, substrate); every voxel on several faces gets the value of the
+// last face in the order x-, x+, y-, y+, z-, z+ whose condition is set (host/dirichlet_solver.cpp:91-110),
+// so threads that reach the same edge voxel from different faces write the same value.
+// Face voxels are enumerated as the two x planes (2*Y*Z), then the two y planes (2*X*Z), then the two z
+// planes (2*X*Y) of the axes the mesh has.
+__global__ __launch_bounds__(256) void dirichlet_faces_kernel(float* __restrict__ dens, const float* __restrict__ fval,
+															  const uint8_t* __restrict__ fcond, int X, int Y, int Z,
+															  int S, int dims, int64_t total)
+{
+	const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= total)
+		return;
+	const int s = (int)(idx % S);
+	int64_t r = idx / S;
+	int x, y, z;
+	const int64_t nx = dims >= 1 ? 2 * (int64_t)Y * Z : 0;
+	const int64_t ny = dims >= 2 ? 2 * (int64_t)X * Z : 0;
+	if (r < nx)
+	{
+		const int64_t side = r / ((int64_t)Y * Z);
+		const int64_t rem = r % ((int64_t)Y * Z);
+		x = side ? X - 1 : 0;
+		y = (int)(rem % Y);
+		z = (int)(rem / Y);
+	}
+	else if (r < nx + ny)
+	{
+		r -= nx;
+		const int64_t side = r / ((int64_t)X * Z);
+		const int64_t rem = r % ((int64_t)X * Z);
+		y = side ? Y - 1 : 0;
+		x = (int)(rem % X);
+		z = (int)(rem / X);
+	}
+	else
+	{
+		r -= nx + ny;
+		const int64_t side = r / ((int64_t)X * Y);
+		const int64_t rem = r % ((int64_t)X * Y);
+		z = side ? Z - 1 : 0;
+		x = (int)(rem % X);
+		y = (int)(rem / X);
+	}
+	const int coord[3] = { x, y, z };
+	const int grid[3] = { X, Y, Z };
+	bool set = false;
+	float v = 0;
+	for (int d = 0; d < dims; d++)
+	{
+		if (coord[d] == 0 && fcond[(2 * d) * S + s])
+		{
+			set = true;
+			v = fval[(2 * d) * S + s];
+		}
+		if (coord[d] == grid[d] - 1 && fcond[(2 * d + 1) * S + s])
+		{
+			set = true;
+			v = fval[(2 * d + 1) * S + s];
+		}
+	}
+	if (set)
+		dens[(((int64_t)z * Y + y) * X + x) * S + s] = v;
+}
+
+// Interior list, de-duplicated per voxel at upload (host/dirichlet_solver.cpp:22-44).  One thread per
+// (entry, substrate).
+__global__ __launch_bounds__(256) void dirichlet_interior_kernel(float* __restrict__ dens,
+																 const int64_t* __restrict__ vox,
+																 const float* __restrict__ val,
+																 const uint8_t* __restrict__ cond, int S,
+																 int64_t total)
+{
+	const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= total)
+		return;
+	const int64_t i = idx / S;
+	const int s = (int)(idx % S);
+	if (cond[idx])
+		dens[vox[i] * S + s] = val[idx];
+}
+
+void launch_dirichlet_faces(bfvm_ctx* ctx)
+{
+	if (!ctx->has_faces)
+		return;
+	const auto& m = ctx->mesh;
+	const int X = m.grid[0], Y = m.grid[1], Z = m.grid[2];
+	int64_t faces = 2 * (int64_t)Y * Z;
+	if (m.dims >= 2)
+		faces += 2 * (int64_t)X * Z;
+	if (m.dims >= 3)
+		faces += 2 * (int64_t)X * Y;
+	const int64_t total = faces * m.S;
+	hipLaunchKernelGGL(dirichlet_faces_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream,
+					   ctx->dens, ctx->face_val, ctx->face_cond, X, Y, Z, m.S, m.dims, total);
+	BFVM_HIP(hipGetLastError());
+}
+
+void launch_dirichlet_interior(bfvm_ctx* ctx)
+{
+	if (ctx->n_interior == 0)
+		return;
+	const int64_t total = ctx->n_interior * ctx->mesh.S;
+	hipLaunchKernelGGL(dirichlet_interior_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream,
+					   ctx->dens, ctx->int_vox, ctx->int_val, ctx->int_cond, ctx->mesh.S, total);
+	BFVM_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------------------
+// gradient — host/gradient_solver.cpp:10-291
+// ------------------------------------------------------------------------------------------------
+
+// One thread per (voxel, substrate).  Interior: (d[i+1] - d[i-1]) * (1.f/(2 dx)); borders:
+// (d[1] - d[0]) * (1.f/dx) and (d[n-1] - d[n-2]) * (1.f/dx), with the float reciprocals of the reference
+// (host/gradient_solver.cpp:17-18).  Output layout [d][z][y][x][s] in 3-D, [y][x][s][d] in 1-D/2-D
+// (src/traits.h:27-33, 55-62, 86-92).
+__global__ __launch_bounds__(256) void gradient_kernel(float* __restrict__ grad, const float* __restrict__ dens, int X,
+													   int Y, int Z, int S, int dims, float bx, float cx, float by,
+													   float cy, float bz, float cz, int64_t total)
+{
+	const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= total)
+		return;
+	const int64_t v = idx / S;
+	const int x = (int)(v % X);
+	const int y = (int)((v / X) % Y);
+	const int z = (int)(v / ((int64_t)X * Y));
+	const int64_t sx = S, sy = (int64_t)X * S, sz = (int64_t)X * Y * S;
+
+	float g[3];
+	{
+		const float* p = dens + idx;
+		if (x == 0)
+			g[0] = (p[sx] - p[0]) * bx;
+		else if (x == X - 1)
+			g[0] = (p[0] - p[-sx]) * bx;
+		else
+			g[0] = (p[sx] - p[-sx]) * cx;
+		if (dims >= 2)
+		{
+			if (y == 0)
+				g[1] = (p[sy] - p[0]) * by;
+			else if (y == Y - 1)
+				g[1] = (p[0] - p[-sy]) * by;
+			else
+				g[1] = (p[sy] - p[-sy]) * cy;
+		}
+		if (dims >= 3)
+		{
+			if (z == 0)
+				g[2] = (p[sz] - p[0]) * bz;
+			else if (z == Z - 1)
+				g[2] = (p[0] - p[-sz]) * bz;
+			else
+				g[2] = (p[sz] - p[-sz]) * cz;
+		}
+	}
+	if (dims == 3)
+	{
+		grad[idx] = g[0];
+		grad[total + idx] = g[1];
+		grad[2 * total + idx] = g[2];
+	}
+	else
+	{
+		for (int d = 0; d < dims; d++)
+			grad[idx * dims + d] = g[d];
+	}
+}
+
+void launch_gradient(bfvm_ctx* ctx)
+{
+	const auto& m = ctx->mesh;
+	const int64_t total = (int64_t)m.V * m.S;
+	float b[3] = { 0, 0, 0 }, c[3] = { 0, 0, 0 };
+	for (int d = 0; d < m.dims; d++)
+	{
+		b[d] = 1.f / m.vs[d];
+		c[d] = 1.f / (m.vs[d] * 2);
+	}
+	hipLaunchKernelGGL(gradient_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, ctx->grad,
+					   ctx->dens, m.grid[0], m.grid[1], m.grid[2], m.S, m.dims, b[0], c[0], b[1], c[1], b[2], c[2],
+					   total);
+	BFVM_HIP(hipGetLastError());
+}
+
+} // namespace bfvm

@@ -1,0 +1,46 @@
+"""The C ABI library loads and exports every symbol include/parabiofvm_hip.h declares (CPU only: no
+compute call is made), and argument checks fail with a status instead of aborting."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "parabiofvm_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(bfvm_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_entry_points():
+    names = declared_symbols()
+    for required in ("bfvm_create", "bfvm_destroy", "bfvm_initialize", "bfvm_store_densities", "bfvm_store_agents",
+                     "bfvm_load_densities", "bfvm_load_gradients", "bfvm_load_internalized", "bfvm_diffusion_solve",
+                     "bfvm_dirichlet_solve", "bfvm_gradient_solve", "bfvm_cell_simulate", "bfvm_cell_release",
+                     "bfvm_timestep", "bfvm_wait_for_all", "bfvm_last_error"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    from parabiofvm_amd import _native
+
+    lib = _native.load()
+    names = declared_symbols()
+    assert names
+    for n in names:
+        assert hasattr(lib, n), n
+    # the ctypes prototypes cover the whole header
+    assert set(names) == set(_native.PROTOTYPES)
+
+
+def test_argument_errors_return_status():
+    from parabiofvm_amd import _native
+
+    lib = _native.load()
+    rc = lib.bfvm_initialize(None, None)
+    assert rc == -1
+    assert b"null" in lib.bfvm_last_error()
+    assert lib.bfvm_create(0, None) == -1
+    assert lib.bfvm_destroy(None) == 0

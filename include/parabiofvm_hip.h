@@ -139,6 +139,13 @@ int bfvm_initialize(bfvm_ctx* ctx, const bfvm_microenvironment_desc* desc);
  * maximum chunk length per thread (0 = automatic).  Takes effect at the next solve. */
 int bfvm_set_sweep_algorithm(bfvm_ctx* ctx, int32_t algorithm, int32_t chunk);
 
+/* Per-axis tuning of the partitioned sweep: register chunk (8..64, 0 = automatic) and lines per workgroup
+ * (y/z: consecutive columns per workgroup, x: rows per workgroup; 0 = automatic).  On a configuration the
+ * register/LDS budget cannot hold, the previous one is kept and an error is returned. */
+int bfvm_set_sweep_tuning(bfvm_ctx* ctx, int32_t axis, int32_t chunk, int32_t lines_per_group);
+/* Workgroups the 3-D gradient kernel aims for by splitting z into chunks (0 = automatic). */
+int bfvm_set_gradient_tuning(bfvm_ctx* ctx, int32_t workgroups);
+
 /* store_data_to_solver / load_data_from_solver parts.  Host pointers, synchronous w.r.t. the host buffer. */
 int bfvm_store_densities(bfvm_ctx* ctx, const float* densities);
 int bfvm_store_agents(bfvm_ctx* ctx, const bfvm_agents_desc* agents);
@@ -170,7 +177,8 @@ int bfvm_store_densities_device(bfvm_ctx* ctx, const float* device_densities);
  * reference-order FP32 Thomas coefficients b' [n][S], c [S], e [n-1][S] computed on the GPU
  * (host/diffusion_solver.cpp:72-142).
  */
-int bfvm_sweep_partition(bfvm_ctx* ctx, int32_t axis, int32_t* blocks, int32_t* block_length);
+int bfvm_sweep_partition(bfvm_ctx* ctx, int32_t axis, int32_t* blocks, int32_t* block_length,
+						 int32_t* lines_per_group);
 int bfvm_exact_coefficients(bfvm_ctx* ctx, int32_t axis, float* b, float* c, float* e);
 
 /* mesh::voxel_position<dims> (src/mesh.cpp:47-53) for `count` positions [count][dims], computed by the GPU
@@ -189,6 +197,7 @@ int bfvm_voxel_positions(bfvm_ctx* ctx, int32_t count, const float* positions, i
 #define BFVM_KERNEL_GRADIENT 3
 #define BFVM_KERNEL_DIRICHLET_INTERIOR 4
 #define BFVM_KERNEL_DIRICHLET_FACES 5
+#define BFVM_KERNEL_COPY 6 /* calibration: float4 copy of the densities into the gradient buffer (8*V*S bytes) */
 #define BFVM_PHASE_SECRETION 10		   /* cell solver, recompute = false */
 #define BFVM_PHASE_SECRETION_RECOMPUTE 11 /* cell solver, recompute = true */
 #define BFVM_PHASE_DIFFUSION 12

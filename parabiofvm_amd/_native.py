@@ -94,7 +94,9 @@ PROTOTYPES = {
     "bfvm_timestep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "bfvm_wait_for_all": (ctypes.c_int, [ctypes.c_void_p]),
     "bfvm_get_device_buffers": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DeviceBuffers)]),
-    "bfvm_sweep_partition": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32, _I32]),
+    "bfvm_sweep_partition": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32, _I32, _I32]),
+    "bfvm_set_sweep_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "bfvm_set_gradient_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "bfvm_exact_coefficients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p]),
     "bfvm_voxel_positions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
@@ -102,6 +104,7 @@ PROTOTYPES = {
 }
 
 KERNELS = {"sweep_x": 0, "sweep_y": 1, "sweep_z": 2, "gradient": 3, "dirichlet_interior": 4, "dirichlet_faces": 5,
+           "copy": 6,
            "secretion": 10, "secretion_recompute": 11, "diffusion": 12, "gradient_phase": 13, "timestep": 14}
 
 _lib = None

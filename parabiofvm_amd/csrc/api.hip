@@ -144,7 +144,7 @@ static void ensure_partitions(bfvm_ctx* ctx)
 {
 	for (int axis = 0; axis < ctx->mesh.dims; axis++)
 	{
-		build_partition(ctx, axis, ctx->chunk_request);
+		build_partition(ctx, axis, ctx->chunk_request[axis], ctx->lines_request[axis]);
 		launch_partition_coefficients(ctx, axis);
 	}
 }
@@ -371,13 +371,64 @@ int bfvm_set_sweep_algorithm(bfvm_ctx* ctx, int32_t algorithm, int32_t chunk)
 			throw std::invalid_argument("chunk must be in [0, 64]");
 		BFVM_HIP(hipSetDevice(ctx->device));
 		ctx->algorithm = algorithm;
-		ctx->chunk_request = chunk;
+		for (int a = 0; a < 3; a++)
+			ctx->chunk_request[a] = chunk;
 		if (ctx->initialized)
 		{
 			BFVM_HIP(hipStreamSynchronize(ctx->stream));
 			ensure_partitions(ctx);
 			BFVM_HIP(hipStreamSynchronize(ctx->stream));
 		}
+		ctx->epoch++;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_set_sweep_tuning(bfvm_ctx* ctx, int32_t axis, int32_t chunk, int32_t lines_per_group)
+{
+	return guarded("bfvm_set_sweep_tuning", [&] {
+		if (!ctx)
+			throw std::invalid_argument("null context");
+		if (axis < 0 || axis > 2)
+			throw std::invalid_argument("axis out of range");
+		if (chunk < 0 || chunk > 64 || lines_per_group < 0 || lines_per_group > 1024)
+			throw std::invalid_argument("chunk must be in [0, 64] and lines_per_group in [0, 1024]");
+		BFVM_HIP(hipSetDevice(ctx->device));
+		const int old_chunk = ctx->chunk_request[axis], old_lines = ctx->lines_request[axis];
+		ctx->chunk_request[axis] = chunk;
+		ctx->lines_request[axis] = lines_per_group;
+		if (ctx->initialized && axis < ctx->mesh.dims)
+		{
+			BFVM_HIP(hipStreamSynchronize(ctx->stream));
+			try
+			{
+				build_partition(ctx, axis, chunk, lines_per_group);
+			}
+			catch (...)
+			{
+				ctx->chunk_request[axis] = old_chunk;
+				ctx->lines_request[axis] = old_lines;
+				build_partition(ctx, axis, old_chunk, old_lines);
+				launch_partition_coefficients(ctx, axis);
+				BFVM_HIP(hipStreamSynchronize(ctx->stream));
+				throw;
+			}
+			launch_partition_coefficients(ctx, axis);
+			BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		}
+		ctx->epoch++;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_set_gradient_tuning(bfvm_ctx* ctx, int32_t workgroups)
+{
+	return guarded("bfvm_set_gradient_tuning", [&] {
+		if (!ctx)
+			throw std::invalid_argument("null context");
+		if (workgroups < 0)
+			throw std::invalid_argument("workgroups must be non-negative");
+		ctx->gradient_groups = workgroups;
 		ctx->epoch++;
 		return BFVM_OK;
 	});
@@ -635,7 +686,8 @@ int bfvm_get_device_buffers(bfvm_ctx* ctx, bfvm_device_buffers* out)
 	});
 }
 
-int bfvm_sweep_partition(bfvm_ctx* ctx, int32_t axis, int32_t* blocks, int32_t* block_length)
+int bfvm_sweep_partition(bfvm_ctx* ctx, int32_t axis, int32_t* blocks, int32_t* block_length,
+						 int32_t* lines_per_group)
 {
 	return guarded("bfvm_sweep_partition", [&] {
 		require_init(ctx);
@@ -646,6 +698,8 @@ int bfvm_sweep_partition(bfvm_ctx* ctx, int32_t axis, int32_t* blocks, int32_t* 
 			*blocks = p.P;
 		if (block_length)
 			*block_length = p.P ? p.base + (p.rem > 0 ? 1 : 0) : 1;
+		if (lines_per_group)
+			*lines_per_group = p.lpg;
 		return BFVM_OK;
 	});
 }
@@ -736,6 +790,9 @@ int bfvm_benchmark(bfvm_ctx* ctx, int32_t what, int32_t iterations, float* avg_m
 				break;
 			case BFVM_KERNEL_DIRICHLET_FACES:
 				launch_dirichlet_faces(ctx);
+				break;
+			case BFVM_KERNEL_COPY:
+				launch_copy(ctx);
 				break;
 			case BFVM_PHASE_SECRETION:
 				launch_cell(ctx, false);

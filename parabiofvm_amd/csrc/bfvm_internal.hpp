@@ -12,6 +12,41 @@
 
 namespace bfvm {
 
+// Vectors of V floats for kernels that move V consecutive substrates per thread (16-byte accesses for V = 4).
+template <int V>
+struct vec_of
+{
+	using type = float __attribute__((ext_vector_type(V)));
+};
+template <>
+struct vec_of<1>
+{
+	using type = float;
+};
+
+template <int V>
+__device__ __forceinline__ typename vec_of<V>::type vload(const float* p)
+{
+	return *(const typename vec_of<V>::type*)p;
+}
+template <int V>
+__device__ __forceinline__ void vstore(float* p, typename vec_of<V>::type v)
+{
+	*(typename vec_of<V>::type*)p = v;
+}
+__device__ __forceinline__ float comp(float v, int) { return v; }
+__device__ __forceinline__ void set_comp(float& v, int, float x) { v = x; }
+template <typename T>
+__device__ __forceinline__ float comp(const T& v, int c)
+{
+	return v[c];
+}
+template <typename T>
+__device__ __forceinline__ void set_comp(T& v, int c, float x)
+{
+	v[c] = x;
+}
+
 void set_error(const std::string& msg);
 
 struct hip_failure
@@ -40,6 +75,9 @@ struct partition
 	int ntypes = 0;
 	int mb = 0; // longest block (row stride of the coefficient table)
 	int chunk = 0; // register capacity of the kernel instance that runs it
+	int lpg = 0;   // lines per workgroup iteration
+	int threads = 0;
+	int vec = 1; // substrates per thread (1 or 4)
 };
 
 // Device-side tables of one axis.
@@ -115,7 +153,9 @@ struct bfvm_ctx
 
 	bfvm::axis_tables axes[3];
 	int algorithm = BFVM_SWEEP_PARTITIONED;
-	int chunk_request = 0;
+	int chunk_request[3] = { 0, 0, 0 };
+	int lines_request[3] = { 0, 0, 0 };
+	int gradient_groups = 0; // workgroups the 3-D gradient aims for (0 = automatic)
 
 	bfvm::agents_buffers ag;
 	int* ballots = nullptr;
@@ -129,7 +169,7 @@ namespace bfvm {
 
 // launches (each enqueues on ctx->stream; throw hip_failure on launch errors)
 void launch_exact_coefficients(bfvm_ctx* ctx, int axis);
-void build_partition(bfvm_ctx* ctx, int axis, int chunk);
+void build_partition(bfvm_ctx* ctx, int axis, int chunk, int lines_request);
 void launch_partition_coefficients(bfvm_ctx* ctx, int axis);
 void launch_sweep(bfvm_ctx* ctx, int axis);
 
@@ -137,6 +177,7 @@ void launch_dirichlet_faces(bfvm_ctx* ctx);
 void launch_dirichlet_interior(bfvm_ctx* ctx);
 
 void launch_gradient(bfvm_ctx* ctx);
+void launch_copy(bfvm_ctx* ctx);
 
 void launch_cell(bfvm_ctx* ctx, bool recompute);
 void launch_cell_release(bfvm_ctx* ctx, int index);

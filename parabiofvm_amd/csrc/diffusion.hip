@@ -388,6 +388,10 @@ __global__ __launch_bounds__(256) void sweep_single_kernel(sweep_geometry g, int
 // partitioned sweep
 // ------------------------------------------------------------------------------------------------
 
+// V consecutive substrates of one voxel row are handled by one thread as a vector (V = 4 when the
+// substrate count is a multiple of 4): every global access is then a 16-byte load or store and every
+// coefficient read a 16-byte LDS read (vec_of / vload / vstore in bfvm_internal.hpp).
+
 struct part_args
 {
 	sweep_geometry g;
@@ -395,22 +399,61 @@ struct part_args
 	const float* coef;
 	const float* red;
 	const float* A;
-	int64_t lines;		 // number of lines
+	int64_t lines;		 // number of lines (groups of V substrates)
 	int lines_per_group; // lines handled by one workgroup iteration
 	int64_t groups;
 };
 
+// Line geometry with V substrates per line.  x: line = (row, substrate group); y: line = (z, q0/V) with
+// q0 = x*S + s0; z: line = q0/V over the whole (y, x, s) plane.
+template <int AXIS, int V>
+__device__ __forceinline__ void vline_coords(const sweep_geometry& g, int64_t line, int64_t& base, int& s0, int* coord)
+{
+	const int64_t XS = (int64_t)g.X * g.S;
+	if constexpr (AXIS == 0)
+	{
+		const int SG = g.S / V;
+		const int64_t row = line / SG;
+		s0 = (int)(line % SG) * V;
+		base = row * XS + s0;
+		coord[0] = 0;
+		coord[1] = (int)(row % g.Y);
+		coord[2] = (int)(row / g.Y);
+	}
+	else if constexpr (AXIS == 1)
+	{
+		const int64_t XSV = XS / V;
+		const int64_t z = line / XSV;
+		const int64_t q0 = (line % XSV) * V;
+		s0 = (int)(q0 % g.S);
+		base = z * (int64_t)g.Y * XS + q0;
+		coord[0] = (int)(q0 / g.S);
+		coord[1] = 0;
+		coord[2] = (int)z;
+	}
+	else
+	{
+		const int64_t q0 = line * V;
+		s0 = (int)(q0 % g.S);
+		base = q0;
+		coord[0] = (int)((q0 / g.S) % g.X);
+		coord[1] = (int)(q0 / XS);
+		coord[2] = 0;
+	}
+}
+
 // Thread t of a workgroup handles block k of line `line_local`:
-//  x: t = (line_local / S) * (S*P) + k*S + s  -> lanes of one row read S consecutive floats per block
-//  y, z: t = k * lines_per_group + line_local -> lanes read consecutive floats of one plane row
+//  x: t = (line_local / SG) * (SG*P) + k*SG + sg -> the lanes of one x row read SG*V consecutive floats per
+//     block (SG = S/V substrate groups)
+//  y, z: t = k * lines_per_group + line_local   -> lanes read consecutive 16-byte vectors of a plane row
 template <int AXIS>
-__device__ __forceinline__ void thread_slot(int t, int S, int P, int lpg, int& line_local, int& k)
+__device__ __forceinline__ void thread_slot(int t, int SG, int P, int lpg, int& line_local, int& k)
 {
 	if constexpr (AXIS == 0)
 	{
-		const int s = t % S;
-		k = (t / S) % P;
-		line_local = (t / (S * P)) * S + s;
+		const int sg = t % SG;
+		k = (t / SG) % P;
+		line_local = (t / (SG * P)) * SG + sg;
 		if (t >= lpg * P)
 			k = -1;
 	}
@@ -423,9 +466,39 @@ __device__ __forceinline__ void thread_slot(int t, int S, int P, int lpg, int& l
 	}
 }
 
-template <int M, int AXIS>
-__global__ __launch_bounds__(1024) void sweep_partitioned_kernel(part_args a)
+// Dirichlet faces for the elements a thread stores: the common case (an interior line away from its ends)
+// stores the solved value; lines on a face of another axis, and the two ends of the line, resolve every
+// face in the order x-, x+, y-, y+, z-, z+ (host/dirichlet_solver.cpp:91-110).
+template <int AXIS, int V>
+__device__ __forceinline__ typename vec_of<V>::type faces_slow(typename vec_of<V>::type x, const sweep_geometry& g,
+															  int s0, const int* coord, int i)
 {
+	const int grid[3] = { g.X, g.Y, g.Z };
+	for (int c = 0; c < V; c++)
+	{
+		const int s = s0 + c;
+		float val = comp(x, c);
+		for (int d = 0; d < g.dims; d++)
+		{
+			const int cd = d == AXIS ? i : coord[d];
+			if (cd == 0 && g.fcond[(2 * d) * g.S + s])
+				val = g.fval[(2 * d) * g.S + s];
+			if (cd == grid[d] - 1 && g.fcond[(2 * d + 1) * g.S + s])
+				val = g.fval[(2 * d + 1) * g.S + s];
+		}
+		set_comp(x, c, val);
+	}
+	return x;
+}
+
+// Workgroups of the 4-substrate kernels are capped at 512 threads so each thread may hold up to 256 VGPRs
+// (a block of 16 float4 values plus coefficients does not fit the 128 of a 1024-thread workgroup).
+constexpr int max_group_threads(int V) { return V == 4 ? 512 : 1024; }
+
+template <int M, int AXIS, int V>
+__global__ __launch_bounds__(max_group_threads(V)) void sweep_partitioned_kernel(part_args a)
+{
+	using vec = typename vec_of<V>::type;
 	extern __shared__ float lds[];
 	const sweep_geometry& g = a.g;
 	const int S = g.S;
@@ -435,7 +508,7 @@ __global__ __launch_bounds__(1024) void sweep_partitioned_kernel(part_args a)
 	const int nred = 3 * 2 * P * S;
 	float* lcoef = lds;
 	float* lred = lds + ncoef;
-	float* rv = lred + nred; // [2P][lpg]
+	vec* rv = (vec*)(lred + nred); // [2P][lpg]
 
 	for (int i = threadIdx.x; i < ncoef; i += blockDim.x)
 		lcoef[i] = a.coef[i];
@@ -443,7 +516,7 @@ __global__ __launch_bounds__(1024) void sweep_partitioned_kernel(part_args a)
 		lred[i] = a.red[i];
 
 	int line_local, k;
-	thread_slot<AXIS>(threadIdx.x, S, P, lpg, line_local, k);
+	thread_slot<AXIS>(threadIdx.x, S / V, P, lpg, line_local, k);
 	const int64_t stride = line_stride<AXIS>(g);
 	const int grid[3] = { g.X, g.Y, g.Z };
 	const size_t fstride = (size_t)a.ntypes * a.mb * S;
@@ -463,31 +536,31 @@ __global__ __launch_bounds__(1024) void sweep_partitioned_kernel(part_args a)
 		const int64_t line = group * lpg + line_local;
 		const bool active = k >= 0 && line < a.lines;
 		int64_t base = 0;
-		int s = 0;
+		int s0 = 0;
 		int coord[3] = { 0, 0, 0 };
 		if (active)
-			line_coords<AXIS>(g, line, base, s, coord);
+			vline_coords<AXIS, V>(g, line, base, s0, coord);
 		float* p = g.data + base + (int64_t)start * stride;
-		const float* cr = lcoef + 0 * fstride + (size_t)type * a.mb * S + s;
-		const float* cg = lcoef + 1 * fstride + (size_t)type * a.mb * S + s;
-		const float* ca = lcoef + 2 * fstride + (size_t)type * a.mb * S + s;
-		const float* cgp = lcoef + 3 * fstride + (size_t)type * a.mb * S + s;
+		const float* cr = lcoef + 0 * fstride + (size_t)type * a.mb * S + s0;
+		const float* cg = lcoef + 1 * fstride + (size_t)type * a.mb * S + s0;
+		const float* ca = lcoef + 2 * fstride + (size_t)type * a.mb * S + s0;
+		const float* cgp = lcoef + 3 * fstride + (size_t)type * a.mb * S + s0;
 
-		float v[M];
+		vec v[M];
 		if (active)
 		{
 #pragma unroll
 			for (int j = 0; j < M; j++)
 				if (j < mk)
-					v[j] = p[j * stride];
+					v[j] = vload<V>(p + j * stride);
 			// forward elimination inside the block
-			const float As = a.A[s];
+			const vec As = vload<V>(a.A + s0);
 #pragma unroll
 			for (int j = 0; j < M; j++)
 			{
 				if (j < mk)
 				{
-					const float r = cr[j * S];
+					const vec r = vload<V>(cr + j * S);
 					if (j < 2)
 						v[j] = r * v[j];
 					else
@@ -498,58 +571,95 @@ __global__ __launch_bounds__(1024) void sweep_partitioned_kernel(part_args a)
 #pragma unroll
 			for (int j = M - 3; j >= 1; j--)
 				if (j <= mk - 3)
-					v[j] = v[j] - cg[j * S] * v[j + 1];
+					v[j] = v[j] - vload<V>(cg + j * S) * v[j + 1];
 			if (mk >= 3)
-				v[0] = ca[0] * (v[0] - cg[0] * v[1]);
+				v[0] = vload<V>(ca) * (v[0] - vload<V>(cg) * v[1]);
 			rv[(2 * k) * lpg + line_local] = v[0];
 #pragma unroll
 			for (int j = 1; j < M; j++)
 				if (j == mk - 1)
 					rv[(2 * k + 1) * lpg + line_local] = v[j];
-			if (mk == 1)
-				rv[(2 * k + 1) * lpg + line_local] = v[0];
 		}
 		__syncthreads();
-		// reduced system of the line, one thread per line
+		// reduced system of the line, one thread per line; the LDS reads of step q+1 are issued before the
+		// arithmetic of step q so the recurrence waits on arithmetic latency only
 		if (active && k == 0)
 		{
-			const float* L = lred + 0 * rstride + s;
-			const float* inv = lred + 1 * rstride + s;
-			const float* cp = lred + 2 * rstride + s;
-			float* r = rv + line_local;
-			float Rp = r[0];
-			for (int q = 1; q < 2 * P; q++)
+			const float* L = lred + 0 * rstride + s0;
+			const float* inv = lred + 1 * rstride + s0;
+			const float* cp = lred + 2 * rstride + s0;
+			vec* r = rv + line_local;
+			const int nq = 2 * P;
+			vec Rp = r[0];
+			vec Rn = r[lpg], Ln = vload<V>(L + S), In = vload<V>(inv + S);
+#pragma unroll 1
+			for (int q = 1; q < nq; q++)
 			{
-				Rp = (r[q * lpg] - L[q * S] * Rp) * inv[q * S];
+				const vec Rq = Rn, Lq = Ln, Iq = In;
+				if (q + 1 < nq)
+				{
+					Rn = r[(q + 1) * lpg];
+					Ln = vload<V>(L + (q + 1) * S);
+					In = vload<V>(inv + (q + 1) * S);
+				}
+				Rp = (Rq - Lq * Rp) * Iq;
 				r[q * lpg] = Rp;
 			}
-			float u = Rp;
-			for (int q = 2 * P - 2; q >= 0; q--)
+			vec u = Rp;
+			vec Rb = r[(nq - 2) * lpg], Cb = vload<V>(cp + (nq - 2) * S);
+#pragma unroll 1
+			for (int q = nq - 2; q >= 0; q--)
 			{
-				u = r[q * lpg] - cp[q * S] * u;
+				const vec Rq = Rb, Cq = Cb;
+				if (q > 0)
+				{
+					Rb = r[(q - 1) * lpg];
+					Cb = vload<V>(cp + (q - 1) * S);
+				}
+				u = Rq - Cq * u;
 				r[q * lpg] = u;
 			}
 		}
 		__syncthreads();
 		if (active)
 		{
-			const float u0 = rv[(2 * k) * lpg + line_local];
-			const float u1 = rv[(2 * k + 1) * lpg + line_local];
-			face_epilogue fe;
-			fe.init(g.fval, g.fcond, S, s, AXIS, coord, grid, g.dims);
+			const vec u0 = rv[(2 * k) * lpg + line_local];
+			const vec u1 = rv[(2 * k + 1) * lpg + line_local];
+			// faces touching this line: another axis's face through the whole line, or this axis's faces at
+			// the line ends
+			bool line_face = false, end_min = false, end_max = false;
+			if (g.fcond)
+			{
+				for (int c = 0; c < V; c++)
+					for (int d = 0; d < g.dims; d++)
+					{
+						const bool cmin = g.fcond[(2 * d) * S + s0 + c] != 0;
+						const bool cmax = g.fcond[(2 * d + 1) * S + s0 + c] != 0;
+						if (d == AXIS)
+						{
+							end_min |= cmin;
+							end_max |= cmax;
+						}
+						else
+							line_face |= (cmin && coord[d] == 0) || (cmax && coord[d] == grid[d] - 1);
+					}
+			}
 #pragma unroll
 			for (int j = 0; j < M; j++)
 			{
 				if (j < mk)
 				{
-					float x;
+					vec x;
 					if (j == 0)
 						x = u0;
 					else if (j == mk - 1)
 						x = u1;
 					else
-						x = v[j] - ca[j * S] * u0 - cgp[j * S] * u1;
-					p[j * stride] = fe.apply(x, start + j, a.n);
+						x = v[j] - vload<V>(ca + j * S) * u0 - vload<V>(cgp + j * S) * u1;
+					const int i = start + j;
+					if (line_face || (i == 0 && end_min) || (i == a.n - 1 && end_max))
+						x = faces_slow<AXIS, V>(x, g, s0, coord, i);
+					vstore<V>(p + j * stride, x);
 				}
 			}
 		}
@@ -606,30 +716,34 @@ static void free_partition_tables(axis_tables& t)
 	t.coef = t.red = t.A = nullptr;
 }
 
-// lines handled by one workgroup and its thread count for a partition of P blocks
-template <int AXIS>
-static void group_shape(int S, int P, int& lpg, int& threads)
+// Lines handled by one workgroup and its thread count for a partition of P blocks.  `request` is the
+// number of lines per workgroup for y/z (16, 32 or 64 consecutive columns; 0 = 32) and the number of
+// x rows per workgroup for x (0 = as many rows as fit in 256 threads).
+static void group_shape(int axis, int SG, int P, int request, int max_threads, int& lpg, int& threads)
 {
-	if (AXIS == 0)
+	if (axis == 0)
 	{
-		const int per_row = S * P;
-		const int rows = std::max(1, 256 / per_row);
-		lpg = rows * S;
+		const int per_row = SG * P; // SG = substrate groups of V per x row
+		const int rows = request > 0 ? request : std::max(1, 256 / per_row);
+		lpg = rows * SG;
 		threads = ((rows * per_row + 63) / 64) * 64;
 	}
 	else
 	{
-		lpg = (64 * P <= 1024) ? 64 : 32;
-		threads = lpg * P;
+		lpg = request > 0 ? request : (max_threads >= 1024 ? 32 : 16);
+		if (lpg * P > max_threads)
+			lpg = std::max(8, max_threads / P / 8 * 8);
+		threads = ((lpg * P + 63) / 64) * 64;
 	}
 }
 
 static size_t partition_lds_bytes(const partition& p, int S, int lpg)
 {
-	return sizeof(float) * ((size_t)4 * p.ntypes * p.mb * S + (size_t)6 * p.P * S + (size_t)2 * p.P * lpg);
+	return sizeof(float) *
+		   ((size_t)4 * p.ntypes * p.mb * S + (size_t)6 * p.P * S + (size_t)2 * p.P * lpg * p.vec);
 }
 
-void build_partition(bfvm_ctx* ctx, int axis, int chunk)
+void build_partition(bfvm_ctx* ctx, int axis, int chunk, int lines_request)
 {
 	auto& t = ctx->axes[axis];
 	const int n = ctx->mesh.grid[axis];
@@ -639,23 +753,27 @@ void build_partition(bfvm_ctx* ctx, int axis, int chunk)
 	t.part.n = n;
 	if (n <= 1)
 		return;
-	int M = supported_chunk(chunk > 0 ? chunk : 32);
+	// 4 substrates per thread when they pack into 16-byte vectors (register chunk 16: 64 data VGPRs)
+	const bool vec4 = S % 4 == 0;
+	int M = supported_chunk(chunk > 0 ? chunk : (vec4 ? 16 : 32));
 	for (;; M = supported_chunk(M + 1))
 	{
 		partition p;
 		p.n = n;
 		p.chunk = M;
+		p.vec = (vec4 && M == 16) ? 4 : 1;
 		p.P = (n + M - 1) / M;
 		p.base = n / p.P;
 		p.rem = n % p.P;
 		p.ntypes = p.P == 1 ? 1 : 4;
 		p.mb = p.P == 1 ? n : p.base + 1;
 		int lpg, threads;
-		if (axis == 0)
-			group_shape<0>(S, p.P, lpg, threads);
-		else
-			group_shape<1>(S, p.P, lpg, threads);
-		const bool fits = threads <= 1024 && partition_lds_bytes(p, S, lpg) <= 64 * 1024 && (p.P == 1 || p.base >= 3);
+		const int max_threads = p.vec == 4 ? 512 : 1024;
+		group_shape(axis, S / p.vec, p.P, lines_request, max_threads, lpg, threads);
+		p.lpg = lpg;
+		p.threads = threads;
+		const bool fits = threads <= max_threads && lpg * p.P <= threads &&
+						  partition_lds_bytes(p, S, lpg) <= 64 * 1024 && (p.P == 1 || p.base >= 3);
 		if (fits || M == 64)
 		{
 			if (!fits)
@@ -684,7 +802,7 @@ void launch_partition_coefficients(bfvm_ctx* ctx, int axis)
 	BFVM_HIP(hipGetLastError());
 }
 
-template <int M, int AXIS>
+template <int M, int AXIS, int V>
 static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const axis_tables& t)
 {
 	const partition& p = t.part;
@@ -700,37 +818,41 @@ static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const a
 	a.red = t.red;
 	a.A = t.A;
 	const int64_t total = (int64_t)g.X * g.Y * g.Z * g.S;
-	a.lines = total / p.n;
-	int lpg, threads;
-	group_shape<AXIS>(g.S, p.P, lpg, threads);
+	a.lines = total / ((int64_t)p.n * V);
+	const int lpg = p.lpg, threads = p.threads;
 	a.lines_per_group = lpg;
 	a.groups = (a.lines + lpg - 1) / lpg;
 	const size_t lds = partition_lds_bytes(p, g.S, lpg);
 	const int64_t max_blocks = 8192;
 	const int blocks = (int)std::min<int64_t>(a.groups, max_blocks);
-	hipLaunchKernelGGL((sweep_partitioned_kernel<M, AXIS>), dim3(blocks), dim3(threads), lds, ctx->stream, a);
+	hipLaunchKernelGGL((sweep_partitioned_kernel<M, AXIS, V>), dim3(blocks), dim3(threads), lds, ctx->stream, a);
 	BFVM_HIP(hipGetLastError());
 }
 
 template <int AXIS>
 static void launch_partitioned_axis(bfvm_ctx* ctx, const sweep_geometry& g, const axis_tables& t)
 {
+	if (t.part.vec == 4)
+	{
+		launch_partitioned_t<16, AXIS, 4>(ctx, g, t);
+		return;
+	}
 	switch (t.part.chunk)
 	{
 	case 8:
-		launch_partitioned_t<8, AXIS>(ctx, g, t);
+		launch_partitioned_t<8, AXIS, 1>(ctx, g, t);
 		break;
 	case 16:
-		launch_partitioned_t<16, AXIS>(ctx, g, t);
+		launch_partitioned_t<16, AXIS, 1>(ctx, g, t);
 		break;
 	case 32:
-		launch_partitioned_t<32, AXIS>(ctx, g, t);
+		launch_partitioned_t<32, AXIS, 1>(ctx, g, t);
 		break;
 	case 48:
-		launch_partitioned_t<48, AXIS>(ctx, g, t);
+		launch_partitioned_t<48, AXIS, 1>(ctx, g, t);
 		break;
 	default:
-		launch_partitioned_t<64, AXIS>(ctx, g, t);
+		launch_partitioned_t<64, AXIS, 1>(ctx, g, t);
 		break;
 	}
 }

@@ -1,4 +1,6 @@
 // Dirichlet conditions and central-difference gradients on gfx950.
+#include <algorithm>
+
 #include "bfvm_internal.hpp"
 
 namespace bfvm {
@@ -177,6 +179,104 @@ __global__ __launch_bounds__(256) void gradient_kernel(float* __restrict__ grad,
 	}
 }
 
+// 3-D gradient, one thread per (y, q) column group (q = x*S + s, V consecutive substrates of one voxel per
+// thread) marching along z with the z-1, z, z+1 densities in registers; x and y neighbours are read from the
+// current plane.  Tiles of 256 threads are numbered so that consecutive tiles (neighbouring y rows) run on
+// the same XCD and share its L2: the dispatcher hands workgroup b to XCD b % 8, so workgroup b takes tile
+// (b % 8) * per_xcd + b / 8.  Index arithmetic is 32-bit per plane; plane offsets are 64-bit.
+constexpr int grad_tile = 256;
+constexpr int num_xcds = 8;
+
+template <int V>
+__global__ __launch_bounds__(grad_tile) void gradient3d_kernel(float* __restrict__ grad, const float* __restrict__ dens,
+															   int Y, int Z, int S, int XS, int tiles_per_row,
+															   int zchunks, int zchunk, int ntiles, int per_xcd,
+															   float bx, float cx, float by, float cy, float bz,
+															   float cz, int64_t VS)
+{
+	using vec = typename vec_of<V>::type;
+	const int b = blockIdx.x;
+	const int tile = (b % num_xcds) * per_xcd + b / num_xcds;
+	if (tile >= ntiles)
+		return;
+	// z chunk outermost: the tiles one XCD runs together are neighbouring rows of the same z range
+	const int row_tiles = ntiles / zchunks;
+	const int zc = tile / row_tiles;
+	const int row_tile = tile % row_tiles;
+	const int y = row_tile / tiles_per_row;
+	const int q = ((row_tile % tiles_per_row) * grad_tile + threadIdx.x) * V;
+	if (q >= XS)
+		return;
+	const int z0 = zc * zchunk;
+	const int z1 = min(Z, z0 + zchunk);
+	const int64_t plane = (int64_t)Y * XS;
+	const int64_t off = (int64_t)y * XS + q;
+	const float* col = dens + off;
+	float* g0 = grad + off;
+	float* g1 = g0 + VS;
+	float* g2 = g1 + VS;
+	const bool xlo = q < S, xhi = q >= XS - S;
+	const bool ylo = y == 0, yhi = y == Y - 1;
+
+	vec zm = z0 > 0 ? vload<V>(col + (int64_t)(z0 - 1) * plane) : vec(0.f);
+	vec c = vload<V>(col + (int64_t)z0 * plane);
+	for (int z = z0; z < z1; z++)
+	{
+		const int64_t o = (int64_t)z * plane;
+		const float* p = col + o;
+		const vec zp = z + 1 < Z ? vload<V>(p + plane) : vec(0.f);
+		const vec xp = xhi ? vec(0.f) : vload<V>(p + S);
+		const vec xm = xlo ? vec(0.f) : vload<V>(p - S);
+		const vec yp = yhi ? vec(0.f) : vload<V>(p + XS);
+		const vec ym = ylo ? vec(0.f) : vload<V>(p - XS);
+		vec gx, gy, gz;
+		if (xlo)
+			gx = (xp - c) * bx;
+		else if (xhi)
+			gx = (c - xm) * bx;
+		else
+			gx = (xp - xm) * cx;
+		if (ylo)
+			gy = (yp - c) * by;
+		else if (yhi)
+			gy = (c - ym) * by;
+		else
+			gy = (yp - ym) * cy;
+		if (z == 0)
+			gz = (zp - c) * bz;
+		else if (z == Z - 1)
+			gz = (c - zm) * bz;
+		else
+			gz = (zp - zm) * cz;
+		vstore<V>(g0 + o, gx);
+		vstore<V>(g1 + o, gy);
+		vstore<V>(g2 + o, gz);
+		zm = c;
+		c = zp;
+	}
+}
+
+// Calibration kernel: float4 copy of n4 16-byte elements.  Its bandwidth is the practical ceiling the
+// solver kernels are compared against, and its counter readings scale the traffic counters.
+__global__ __launch_bounds__(256) void copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src,
+												   int64_t n4)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+		dst[i] = src[i];
+}
+
+void launch_copy(bfvm_ctx* ctx)
+{
+	const int64_t n = (int64_t)ctx->mesh.V * ctx->mesh.S;
+	const int64_t n4 = n / 4;
+	if (n4 == 0)
+		return;
+	hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, ctx->stream, (float4*)ctx->grad,
+					   (const float4*)ctx->dens, n4);
+	BFVM_HIP(hipGetLastError());
+}
+
 void launch_gradient(bfvm_ctx* ctx)
 {
 	const auto& m = ctx->mesh;
@@ -186,6 +286,27 @@ void launch_gradient(bfvm_ctx* ctx)
 	{
 		b[d] = 1.f / m.vs[d];
 		c[d] = 1.f / (m.vs[d] * 2);
+	}
+	if (m.dims == 3)
+	{
+		const int X = m.grid[0], Y = m.grid[1], Z = m.grid[2];
+		const int XS = X * m.S;
+		const int V = m.S % 4 == 0 ? 4 : 1;
+		const int tiles_per_row = (XS / V + grad_tile - 1) / grad_tile;
+		// split z so that there are enough workgroups to fill the chip several times over
+		const int64_t row_tiles = (int64_t)Y * tiles_per_row;
+		const int64_t target = ctx->gradient_groups > 0 ? ctx->gradient_groups : 16384;
+		int zchunks = (int)std::max<int64_t>(1, std::min<int64_t>(Z, (target + row_tiles - 1) / row_tiles));
+		const int zchunk = (Z + zchunks - 1) / zchunks;
+		zchunks = (Z + zchunk - 1) / zchunk;
+		const int64_t ntiles = row_tiles * zchunks;
+		const int per_xcd = (int)((ntiles + num_xcds - 1) / num_xcds);
+		auto kernel = V == 4 ? gradient3d_kernel<4> : gradient3d_kernel<1>;
+		hipLaunchKernelGGL(kernel, dim3((unsigned)(per_xcd * num_xcds)), dim3(grad_tile), 0, ctx->stream, ctx->grad,
+						   ctx->dens, Y, Z, m.S, XS, tiles_per_row, zchunks, zchunk, (int)ntiles, per_xcd, b[0], c[0],
+						   b[1], c[1], b[2], c[2], total);
+		BFVM_HIP(hipGetLastError());
+		return;
 	}
 	hipLaunchKernelGGL(gradient_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, ctx->grad,
 					   ctx->dens, m.grid[0], m.grid[1], m.grid[2], m.S, m.dims, b[0], c[0], b[1], c[1], b[2], c[2],

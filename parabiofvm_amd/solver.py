@@ -190,10 +190,20 @@ class DeviceSolver:
         return b
 
     def partition(self, axis):
+        """(blocks, block length, lines per workgroup) of the partitioned sweep along `axis`."""
         p = ctypes.c_int32()
         m = ctypes.c_int32()
-        check("bfvm_sweep_partition", self.lib.bfvm_sweep_partition(self.h, axis, ctypes.byref(p), ctypes.byref(m)))
-        return p.value, m.value
+        lpg = ctypes.c_int32()
+        check("bfvm_sweep_partition", self.lib.bfvm_sweep_partition(self.h, axis, ctypes.byref(p), ctypes.byref(m),
+                                                                    ctypes.byref(lpg)))
+        return p.value, m.value, lpg.value
+
+    def set_gradient_tuning(self, workgroups):
+        check("bfvm_set_gradient_tuning", self.lib.bfvm_set_gradient_tuning(self.h, int(workgroups)))
+
+    def set_sweep_tuning(self, axis, chunk, lines_per_group):
+        check("bfvm_set_sweep_tuning", self.lib.bfvm_set_sweep_tuning(self.h, int(axis), int(chunk),
+                                                                      int(lines_per_group)))
 
     def exact_coefficients(self, axis):
         n = self.grid[axis]

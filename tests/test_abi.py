@@ -44,3 +44,15 @@ def test_argument_errors_return_status():
     assert b"null" in lib.bfvm_last_error()
     assert lib.bfvm_create(0, None) == -1
     assert lib.bfvm_destroy(None) == 0
+
+
+def test_cpp_facade_built_and_linked():
+    """The C++ drop-in facade and its test driver link against the in-tree HIP library."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "parabiofvm_amd", "cpp", "build", "test_device_solver")
+    if not os.path.exists(exe):
+        pytest.skip("C++ facade not built (run __graft_entry__.build())")
+    out = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
+    assert "libbiofvm_device.so" in out and "libparabiofvm_hip.so" in out
+    assert "not found" not in out

@@ -125,6 +125,7 @@ SHAPES = [
     ("2d", 2, (150, 37, 1), 5),
     ("1d", 1, (129, 1, 1), 3),
     ("3d_wide_s", 3, (40, 12, 10), 16),
+    ("3d_vec4", 3, (64, 40, 36), 8),
 ]
 
 
@@ -349,3 +350,17 @@ def test_gpu_timestep_graph(algorithm, pkg, oracle_mod):
     dmax = max(1.0, float(np.abs(o.densities).max()), float(np.abs(cfg["init"]).max()))
     assert_internalized(g.internalized, o.internalized, cfg["agents"], cfg["S"], cfg["dt"], 10, dmax,
                         what="internalized")
+
+
+@pytest.mark.parametrize("axis,chunk,lines", [(0, 16, 1), (0, 32, 3), (0, 64, 2), (1, 16, 16), (1, 48, 64),
+                                              (2, 32, 16), (2, 64, 32), (2, 8, 16)])
+def test_gpu_partitioned_tuning(axis, chunk, lines, pkg, oracle_mod):
+    """Every per-axis (chunk, lines per workgroup) the tuner may pick gives the same parity."""
+    p = random_problem(3, (70, 45, 33), 3, seed=61)
+    o, g = build_pair(pkg, oracle_mod, p)
+    g.set_sweep_tuning(axis, chunk, lines)
+    g.store()
+    o.diffusion()
+    g.diffusion()
+    g.load()
+    assert_close(g.densities, o.densities, what=f"axis {axis} chunk {chunk} lines {lines}: {g.partition(axis)}")

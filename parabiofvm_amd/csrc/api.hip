@@ -71,6 +71,8 @@ static void free_agents(bfvm_ctx* ctx)
 	for (float** p : { &a.sec, &a.sat, &a.upt, &a.nex, &a.intern, &a.frac, &a.vol, &a.pos, &a.num, &a.den, &a.fac,
 					   &a.rnum, &a.rden, &a.rfac })
 		dfree(*p);
+	dfree(a.vox);
+	dfree(a.win);
 	a.capacity = 0;
 	a.count = 0;
 }
@@ -490,6 +492,8 @@ int bfvm_store_agents(bfvm_ctx* ctx, const bfvm_agents_desc* agents)
 			}
 			dalloc(a.vol, A);
 			dalloc(a.pos, A * dims);
+			dalloc(a.vox, A);
+			dalloc(a.win, A);
 		}
 		if ((int32_t)A != a.count)
 			ctx->epoch++;
@@ -510,6 +514,8 @@ int bfvm_store_agents(bfvm_ctx* ctx, const bfvm_agents_desc* agents)
 				BFVM_HIP(hipMemsetAsync(a.frac, 0, sizeof(float) * A * S, ctx->stream));
 			up(a.vol, agents->volumes, A);
 			up(a.pos, agents->positions, A * dims);
+			// positions change only through this upload: the agent -> voxel table is built here
+			launch_cell_voxels(ctx);
 		}
 		BFVM_HIP(hipStreamSynchronize(ctx->stream));
 		return BFVM_OK;

@@ -103,6 +103,8 @@ struct agents_buffers
 	float *vol = nullptr, *pos = nullptr;
 	float *num = nullptr, *den = nullptr, *fac = nullptr;
 	float *rnum = nullptr, *rden = nullptr, *rfac = nullptr;
+	int64_t* vox = nullptr; // voxel of each agent (-1 outside the mesh), computed at upload
+	int* win = nullptr;		// ballot winner whose slot collects each agent's terms
 };
 
 struct mesh_info
@@ -180,6 +182,7 @@ void launch_gradient(bfvm_ctx* ctx);
 void launch_copy(bfvm_ctx* ctx);
 
 void launch_cell(bfvm_ctx* ctx, bool recompute);
+void launch_cell_voxels(bfvm_ctx* ctx);
 void launch_cell_release(bfvm_ctx* ctx, int index);
 void launch_voxel_positions(bfvm_ctx* ctx, int count, const float* d_pos, int* d_out);
 

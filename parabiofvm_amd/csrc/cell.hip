@@ -11,8 +11,12 @@
 // two-phase update (internalized with the old D for every agent, then the winners update D) — the same
 // decision the reference makes on the host (host/cell_solver.cpp:208-244).
 //
-// Agents whose position maps outside the grid (or is not finite) are ignored; the reference would index
-// out of bounds (src/mesh.cpp:47-53 does not clamp).
+// Each agent's voxel is computed once when the agents are uploaded (positions change only through an
+// upload) and kept in agent_vox; the per-substrate kernels run one thread per (agent, substrate) so that
+// consecutive lanes touch consecutive floats.
+//
+// Agents whose position maps outside the grid (or is not finite) get voxel -1 and are ignored; the
+// reference would index out of bounds (src/mesh.cpp:47-53 does not clamp).
 #include "bfvm_internal.hpp"
 
 namespace bfvm {
@@ -41,80 +45,99 @@ __device__ __forceinline__ int64_t agent_voxel(const cell_geom& g, const float* 
 	return ((int64_t)idx[2] * g.grid[1] + idx[1]) * g.grid[0] + idx[0];
 }
 
-// recompute, part 1 (host/cell_solver.cpp:35-85): intermediates, ballot reset, reduced-slot reset.
-__global__ __launch_bounds__(256) void cell_prepare_kernel(cell_geom g, int A, float voxel_volume, float dt,
-														   const float* __restrict__ pos, const float* __restrict__ vol,
-														   const float* __restrict__ sec, const float* __restrict__ sat,
-														   const float* __restrict__ upt, const float* __restrict__ nex,
-														   float* __restrict__ num, float* __restrict__ den,
-														   float* __restrict__ fac, float* __restrict__ rnum,
-														   float* __restrict__ rden, float* __restrict__ rfac,
-														   int* __restrict__ ballots, int* __restrict__ conflict)
+// agent -> voxel table, at upload
+__global__ __launch_bounds__(256) void cell_voxels_kernel(cell_geom g, int A, const float* __restrict__ pos,
+														  int64_t* __restrict__ vox)
 {
 	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i == 0)
+	if (i < A)
+		vox[i] = agent_voxel(g, pos, i);
+}
+
+// recompute, part 1 (host/cell_solver.cpp:35-85): intermediates and reduced-slot reset per (agent, s);
+// ballot reset per agent.
+__global__ __launch_bounds__(256) void cell_prepare_kernel(int A, int S, float voxel_volume, float dt,
+														   const int64_t* __restrict__ vox,
+														   const float* __restrict__ vol, const float* __restrict__ sec,
+														   const float* __restrict__ sat, const float* __restrict__ upt,
+														   const float* __restrict__ nex, float* __restrict__ num,
+														   float* __restrict__ den, float* __restrict__ fac,
+														   float* __restrict__ rnum, float* __restrict__ rden,
+														   float* __restrict__ rfac, int* __restrict__ ballots,
+														   int* __restrict__ conflict)
+{
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k == 0)
 		atomicExch(conflict, 0);
+	if (k >= (int64_t)A * S)
+		return;
+	const int64_t i = k / S;
+	const int s = (int)(k - i * S);
+	if (s == 0)
+	{
+		const int64_t v = vox[i];
+		if (v >= 0)
+			ballots[v] = -1;
+	}
+	const float cv = vol[i];
+	num[k] = sec[k] * sat[k] * dt * cv / voxel_volume;
+	den[k] = (upt[k] + sec[k]) * dt * cv / voxel_volume;
+	fac[k] = nex[k] * dt / voxel_volume;
+	rnum[k] = 0;
+	rden[k] = 0;
+	rfac[k] = 0;
+}
+
+// recompute, part 2 (host/cell_solver.cpp:98-122): claim the voxel; win[i] = the agent whose slot collects i
+__global__ __launch_bounds__(256) void cell_ballot_kernel(int A, const int64_t* __restrict__ vox,
+														  int* __restrict__ ballots, int* __restrict__ win,
+														  int* __restrict__ conflict)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= A)
 		return;
-	const int64_t v = agent_voxel(g, pos, i);
-	if (v >= 0)
-		ballots[v] = -1;
-	const float cv = vol[i];
-	for (int s = 0; s < g.S; s++)
+	const int64_t v = vox[i];
+	if (v < 0)
 	{
-		const int64_t k = i * g.S + s;
-		num[k] = sec[k] * sat[k] * dt * cv / voxel_volume;
-		den[k] = (upt[k] + sec[k]) * dt * cv / voxel_volume;
-		fac[k] = nex[k] * dt / voxel_volume;
-		rnum[k] = 0;
-		rden[k] = 0;
-		rfac[k] = 0;
+		win[i] = -1;
+		return;
+	}
+	const int prev = atomicCAS(&ballots[v], -1, (int)i);
+	if (prev == -1)
+		win[i] = (int)i;
+	else
+	{
+		win[i] = prev;
+		atomicOr(conflict, 1);
 	}
 }
 
-// recompute, part 2 (host/cell_solver.cpp:87-135): claim the voxel, add into the winner's slot.
-__global__ __launch_bounds__(256) void cell_ballot_kernel(cell_geom g, int A, const float* __restrict__ pos,
-														  const float* __restrict__ num, const float* __restrict__ den,
-														  const float* __restrict__ fac, float* __restrict__ rnum,
-														  float* __restrict__ rden, float* __restrict__ rfac,
-														  int* __restrict__ ballots, int* __restrict__ conflict)
+// recompute, part 3 (host/cell_solver.cpp:108-133): every (agent, s) adds into its winner's slot; the
+// winner adds den + 1.
+__global__ __launch_bounds__(256) void cell_sum_kernel(int A, int S, const int* __restrict__ win,
+													   const float* __restrict__ num, const float* __restrict__ den,
+													   const float* __restrict__ fac, float* __restrict__ rnum,
+													   float* __restrict__ rden, float* __restrict__ rfac)
 {
-	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= A)
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= (int64_t)A * S)
 		return;
-	const int64_t v = agent_voxel(g, pos, i);
-	if (v < 0)
+	const int64_t i = k / S;
+	const int s = (int)(k - i * S);
+	const int w = win[i];
+	if (w < 0)
 		return;
-	const int prev = atomicCAS(&ballots[v], -1, (int)i);
-	if (prev == -1)
-	{
-		for (int s = 0; s < g.S; s++)
-		{
-			const int64_t k = i * g.S + s;
-			atomicAdd(&rnum[k], num[k]);
-			atomicAdd(&rden[k], den[k] + 1);
-			atomicAdd(&rfac[k], fac[k]);
-		}
-	}
-	else
-	{
-		atomicOr(conflict, 1);
-		for (int s = 0; s < g.S; s++)
-		{
-			const int64_t k = i * g.S + s;
-			const int64_t w = (int64_t)prev * g.S + s;
-			atomicAdd(&rnum[w], num[k]);
-			atomicAdd(&rden[w], den[k]);
-			atomicAdd(&rfac[w], fac[k]);
-		}
-	}
+	const int64_t t = (int64_t)w * S + s;
+	atomicAdd(&rnum[t], num[k]);
+	atomicAdd(&rden[t], w == (int)i ? den[k] + 1 : den[k]);
+	atomicAdd(&rfac[t], fac[k]);
 }
 
 // Apply, part 1 (host/cell_solver.cpp:137-152, 173-195, 208-234).  One thread per (agent, substrate).
 // No conflict and internalized: the fused update of I and D.  Conflict and internalized: I with the
 // agent's own terms and the current D (D is updated by part 2 after the kernel boundary).
-__global__ __launch_bounds__(256) void cell_apply_kernel(cell_geom g, int A, int with_internalized, int voxel_volume,
-														 const float* __restrict__ pos, float* __restrict__ dens,
+__global__ __launch_bounds__(256) void cell_apply_kernel(int A, int S, int with_internalized, int voxel_volume,
+														 const int64_t* __restrict__ vox, float* __restrict__ dens,
 														 float* __restrict__ intern, const float* __restrict__ num,
 														 const float* __restrict__ den, const float* __restrict__ fac,
 														 const float* __restrict__ rnum,
@@ -124,34 +147,34 @@ __global__ __launch_bounds__(256) void cell_apply_kernel(cell_geom g, int A, int
 {
 	if (!with_internalized)
 		return;
-	const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx >= (int64_t)A * g.S)
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= (int64_t)A * S)
 		return;
-	const int64_t i = idx / g.S;
-	const int s = (int)(idx % g.S);
-	const int64_t v = agent_voxel(g, pos, i);
+	const int64_t i = k / S;
+	const int s = (int)(k - i * S);
+	const int64_t v = vox[i];
 	if (v < 0)
 		return;
-	float* D = dens + v * g.S + s;
+	float* D = dens + v * S + s;
 	const float vv = (float)voxel_volume;
 	if (*conflict == 0)
 	{
 		const float d = *D;
-		const float rn = rnum[idx], rd = rden[idx], rf = rfac[idx];
-		intern[idx] = intern[idx] - vv * (((1 - rd) * d + rn) / (rd) + rf);
+		const float rn = rnum[k], rd = rden[k], rf = rfac[k];
+		intern[k] = intern[k] - vv * (((1 - rd) * d + rn) / (rd) + rf);
 		*D = (d + rn) / rd + rf;
 	}
 	else
 	{
 		const float d = *D;
-		intern[idx] = intern[idx] - vv * (((-den[idx]) * d + num[idx]) / (1 + den[idx]) + fac[idx]);
+		intern[k] = intern[k] - vv * (((-den[k]) * d + num[k]) / (1 + den[k]) + fac[k]);
 	}
 }
 
 // Apply, part 2 (host/cell_solver.cpp:154-171, 236-244): the ballot winner of each voxel updates D with
 // the reduced sums.  Skipped when part 1 already did the fused update.
-__global__ __launch_bounds__(256) void cell_densities_kernel(cell_geom g, int A, int with_internalized,
-															 const float* __restrict__ pos, float* __restrict__ dens,
+__global__ __launch_bounds__(256) void cell_densities_kernel(int A, int S, int with_internalized,
+															 const int64_t* __restrict__ vox, float* __restrict__ dens,
 															 const float* __restrict__ rnum,
 															 const float* __restrict__ rden,
 															 const float* __restrict__ rfac,
@@ -160,33 +183,33 @@ __global__ __launch_bounds__(256) void cell_densities_kernel(cell_geom g, int A,
 {
 	if (with_internalized && *conflict == 0)
 		return;
-	const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx >= (int64_t)A * g.S)
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= (int64_t)A * S)
 		return;
-	const int64_t i = idx / g.S;
-	const int s = (int)(idx % g.S);
-	const int64_t v = agent_voxel(g, pos, i);
+	const int64_t i = k / S;
+	const int s = (int)(k - i * S);
+	const int64_t v = vox[i];
 	if (v < 0)
 		return;
 	if (ballots[v] != (int)i)
 		return;
-	float* D = dens + v * g.S + s;
-	*D = (*D + rnum[idx]) / rden[idx] + rfac[idx];
+	float* D = dens + v * S + s;
+	*D = (*D + rnum[k]) / rden[k] + rfac[k];
 }
 
 // host/cell_solver.cpp:301-328: D += I*F/vv, I = 0 for one agent.
-__global__ void cell_release_kernel(cell_geom g, int64_t i, float voxel_volume, const float* __restrict__ pos,
+__global__ void cell_release_kernel(int S, int64_t i, float voxel_volume, const int64_t* __restrict__ vox,
 									float* __restrict__ dens, float* __restrict__ intern,
 									const float* __restrict__ frac)
 {
 	const int s = threadIdx.x + blockIdx.x * blockDim.x;
-	if (s >= g.S)
+	if (s >= S)
 		return;
-	const int64_t v = agent_voxel(g, pos, i);
+	const int64_t v = vox[i];
 	if (v < 0)
 		return;
-	const int64_t k = i * g.S + s;
-	dens[v * g.S + s] = dens[v * g.S + s] + intern[k] * frac[k] / voxel_volume;
+	const int64_t k = i * S + s;
+	dens[v * S + s] = dens[v * S + s] + intern[k] * frac[k] / voxel_volume;
 	intern[k] = 0;
 }
 
@@ -223,36 +246,49 @@ static cell_geom make_geom(const bfvm_ctx* ctx)
 
 static int voxel_volume(const bfvm_ctx* ctx) { return ctx->mesh.vs[0] * ctx->mesh.vs[1] * ctx->mesh.vs[2]; }
 
+void launch_cell_voxels(bfvm_ctx* ctx)
+{
+	const auto& a = ctx->ag;
+	if (a.count == 0)
+		return;
+	hipLaunchKernelGGL(cell_voxels_kernel, dim3((a.count + 255) / 256), dim3(256), 0, ctx->stream, make_geom(ctx),
+					   a.count, a.pos, a.vox);
+	BFVM_HIP(hipGetLastError());
+}
+
 void launch_cell(bfvm_ctx* ctx, bool recompute)
 {
 	const auto& a = ctx->ag;
 	const int A = a.count;
-	const cell_geom g = make_geom(ctx);
+	const int S = ctx->mesh.S;
 	const unsigned agent_blocks = (unsigned)((A + 255) / 256);
-	const int64_t pairs = (int64_t)A * g.S;
+	const int64_t pairs = (int64_t)A * S;
 	const unsigned pair_blocks = (unsigned)((pairs + 255) / 256);
 	if (recompute)
 	{
 		// at least one block so the conflict flag is cleared even without agents
-		hipLaunchKernelGGL(cell_prepare_kernel, dim3(agent_blocks > 0 ? agent_blocks : 1), dim3(256), 0, ctx->stream,
-						   g, A, (float)voxel_volume(ctx), ctx->mesh.dt, a.pos, a.vol, a.sec, a.sat, a.upt, a.nex,
-						   a.num, a.den, a.fac, a.rnum, a.rden, a.rfac, ctx->ballots, ctx->conflict);
+		hipLaunchKernelGGL(cell_prepare_kernel, dim3(pair_blocks > 0 ? pair_blocks : 1), dim3(256), 0, ctx->stream, A,
+						   S, (float)voxel_volume(ctx), ctx->mesh.dt, a.vox, a.vol, a.sec, a.sat, a.upt, a.nex, a.num,
+						   a.den, a.fac, a.rnum, a.rden, a.rfac, ctx->ballots, ctx->conflict);
 		BFVM_HIP(hipGetLastError());
 		if (A > 0)
 		{
-			hipLaunchKernelGGL(cell_ballot_kernel, dim3(agent_blocks), dim3(256), 0, ctx->stream, g, A, a.pos, a.num,
-							   a.den, a.fac, a.rnum, a.rden, a.rfac, ctx->ballots, ctx->conflict);
+			hipLaunchKernelGGL(cell_ballot_kernel, dim3(agent_blocks), dim3(256), 0, ctx->stream, A, a.vox,
+							   ctx->ballots, a.win, ctx->conflict);
+			BFVM_HIP(hipGetLastError());
+			hipLaunchKernelGGL(cell_sum_kernel, dim3(pair_blocks), dim3(256), 0, ctx->stream, A, S, a.win, a.num,
+							   a.den, a.fac, a.rnum, a.rden, a.rfac);
 			BFVM_HIP(hipGetLastError());
 		}
 	}
 	if (A == 0)
 		return;
-	hipLaunchKernelGGL(cell_apply_kernel, dim3(pair_blocks), dim3(256), 0, ctx->stream, g, A,
-					   (int)ctx->compute_internalized, voxel_volume(ctx), a.pos, ctx->dens, a.intern, a.num, a.den,
+	hipLaunchKernelGGL(cell_apply_kernel, dim3(pair_blocks), dim3(256), 0, ctx->stream, A, S,
+					   (int)ctx->compute_internalized, voxel_volume(ctx), a.vox, ctx->dens, a.intern, a.num, a.den,
 					   a.fac, a.rnum, a.rden, a.rfac, ctx->conflict);
 	BFVM_HIP(hipGetLastError());
-	hipLaunchKernelGGL(cell_densities_kernel, dim3(pair_blocks), dim3(256), 0, ctx->stream, g, A,
-					   (int)ctx->compute_internalized, a.pos, ctx->dens, a.rnum, a.rden, a.rfac, ctx->ballots,
+	hipLaunchKernelGGL(cell_densities_kernel, dim3(pair_blocks), dim3(256), 0, ctx->stream, A, S,
+					   (int)ctx->compute_internalized, a.vox, ctx->dens, a.rnum, a.rden, a.rfac, ctx->ballots,
 					   ctx->conflict);
 	BFVM_HIP(hipGetLastError());
 }
@@ -260,9 +296,9 @@ void launch_cell(bfvm_ctx* ctx, bool recompute)
 void launch_cell_release(bfvm_ctx* ctx, int index)
 {
 	const auto& a = ctx->ag;
-	const cell_geom g = make_geom(ctx);
-	hipLaunchKernelGGL(cell_release_kernel, dim3((g.S + 63) / 64), dim3(64), 0, ctx->stream, g, (int64_t)index,
-					   (float)voxel_volume(ctx), a.pos, ctx->dens, a.intern, a.frac);
+	const int S = ctx->mesh.S;
+	hipLaunchKernelGGL(cell_release_kernel, dim3((S + 63) / 64), dim3(64), 0, ctx->stream, S, (int64_t)index,
+					   (float)voxel_volume(ctx), a.vox, ctx->dens, a.intern, a.frac);
 	BFVM_HIP(hipGetLastError());
 }
 

@@ -65,6 +65,15 @@ typedef struct bfvm_ctx bfvm_ctx;
 #define BFVM_ERR_HIP (-3)
 #define BFVM_ERR_UNSUPPORTED (-4)
 
+/* Density storage in HBM between kernels (bfvm_set_storage).  All arithmetic is FP32 in registers; the
+ * 16-bit forms round to nearest even at every store.  BF16 stores the value itself; FP16 stores
+ * value * 2^k_s with a per-substrate power of two chosen at each densities upload so that the largest of
+ * the uploaded densities and the Dirichlet values of substrate s lands in [2^10, 2^11) (64x headroom below
+ * FP16's largest finite value before a density would overflow). */
+#define BFVM_STORAGE_F32 0
+#define BFVM_STORAGE_BF16 1
+#define BFVM_STORAGE_F16 2
+
 /* Sweep algorithm (bfvm_set_sweep_algorithm). */
 #define BFVM_SWEEP_PARTITIONED 0 /* register-resident partitioned (modified) Thomas, FP64-derived coefficients */
 #define BFVM_SWEEP_EXACT 1		 /* one thread per line, the reference's FP32 coefficients and operation order */
@@ -116,13 +125,15 @@ typedef struct bfvm_agents_desc
 /* Device addresses of the context's buffers, for harnesses that keep data resident in HBM. */
 typedef struct bfvm_device_buffers
 {
-	float* densities;			   /* [V*S] */
+	void* densities;			   /* [V*S] in the storage form (float for BFVM_STORAGE_F32) */
 	float* gradients;			   /* [dims*V*S] */
 	float* internalized_substrates; /* [capacity*S] */
 	int64_t voxel_count;
 	int32_t grid_shape[3];
 	int32_t substrates_count;
-	void* stream; /* hipStream_t the context enqueues on */
+	void* stream;				 /* hipStream_t the context enqueues on */
+	int32_t storage;			 /* BFVM_STORAGE_* of `densities` */
+	const float* density_scales; /* [S] device: stored = value * density_scales[s] (all 1 unless FP16) */
 } bfvm_device_buffers;
 
 const char* bfvm_last_error(void);
@@ -134,6 +145,12 @@ int bfvm_destroy(bfvm_ctx* ctx);
 /* solver::initialize: allocates densities (filled with zeros) and gradients, uploads the Dirichlet
  * description, computes the Thomas coefficients on the GPU.  May be called again to re-initialise. */
 int bfvm_initialize(bfvm_ctx* ctx, const bfvm_microenvironment_desc* desc);
+
+/* Density storage form (BFVM_STORAGE_*), no counterpart in the reference (its precision switch is the FP64
+ * build, CMakeLists.txt:64-66).  Takes effect at the next bfvm_initialize; default FP32.  The 16-bit forms
+ * run the partitioned sweep only (the exact sweep reports BFVM_ERR_UNSUPPORTED); densities cross the API in
+ * FP32 and are converted on the device. */
+int bfvm_set_storage(bfvm_ctx* ctx, int32_t storage);
 
 /* Selects the sweep algorithm (default BFVM_SWEEP_PARTITIONED) and, for the partitioned one, the
  * maximum chunk length per thread (0 = automatic).  Takes effect at the next solve. */
@@ -169,7 +186,7 @@ int bfvm_wait_for_all(bfvm_ctx* ctx);
 int bfvm_get_device_buffers(bfvm_ctx* ctx, bfvm_device_buffers* out);
 
 /* Device-resident variants for harnesses: same as the host ones but from/to device memory on the
- * context's stream (asynchronous). */
+ * context's stream (asynchronous, except that FP16 storage synchronises once to choose its scales). */
 int bfvm_store_densities_device(bfvm_ctx* ctx, const float* device_densities);
 
 /*

@@ -12,6 +12,10 @@ LIB_PATH = os.path.join(HERE, "libparabiofvm_hip.so")
 
 BFVM_SWEEP_PARTITIONED = 0
 BFVM_SWEEP_EXACT = 1
+BFVM_STORAGE_F32 = 0
+BFVM_STORAGE_BF16 = 1
+BFVM_STORAGE_F16 = 2
+STORAGE = {"f32": BFVM_STORAGE_F32, "bf16": BFVM_STORAGE_BF16, "f16": BFVM_STORAGE_F16}
 
 
 class BfvmError(RuntimeError):
@@ -70,6 +74,8 @@ class DeviceBuffers(ctypes.Structure):
         ("grid_shape", ctypes.c_int32 * 3),
         ("substrates_count", ctypes.c_int32),
         ("stream", ctypes.c_void_p),
+        ("storage", ctypes.c_int32),
+        ("density_scales", ctypes.c_void_p),
     ]
 
 
@@ -79,6 +85,7 @@ PROTOTYPES = {
     "bfvm_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
     "bfvm_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "bfvm_initialize": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MicroenvironmentDesc)]),
+    "bfvm_set_storage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "bfvm_set_sweep_algorithm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "bfvm_store_densities": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "bfvm_store_densities_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

@@ -1,4 +1,6 @@
 // C ABI of libparabiofvm_hip.so (declared in include/parabiofvm_hip.h).
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <unordered_map>
@@ -57,6 +59,13 @@ static void dalloc(T*& p, size_t count)
 		BFVM_HIP(hipMalloc(&p, sizeof(T) * count));
 }
 
+static void dalloc_bytes(void*& p, size_t bytes)
+{
+	dfree(p);
+	if (bytes > 0)
+		BFVM_HIP(hipMalloc(&p, bytes));
+}
+
 static void require_init(const bfvm_ctx* ctx)
 {
 	if (!ctx)
@@ -92,6 +101,12 @@ static void free_graphs(bfvm_ctx* ctx)
 static void free_fields(bfvm_ctx* ctx)
 {
 	dfree(ctx->dens);
+	dfree(ctx->dscale);
+	dfree(ctx->dinv);
+	dfree(ctx->maxbits);
+	dfree(ctx->rounding_step);
+	dfree(ctx->staging);
+	ctx->staging_elems = 0;
 	dfree(ctx->grad);
 	dfree(ctx->d_D);
 	dfree(ctx->d_decay);
@@ -127,6 +142,10 @@ static void enqueue_dirichlet(bfvm_ctx* ctx)
 // faces of the Dirichlet that follows each sweep are applied in that sweep's store epilogue.
 static void enqueue_diffusion(bfvm_ctx* ctx)
 {
+	if (ctx->algorithm == BFVM_SWEEP_EXACT && ctx->storage != BFVM_STORAGE_F32)
+		throw std::runtime_error("the exact sweep needs FP32 density storage");
+	if (ctx->storage != BFVM_STORAGE_F32)
+		launch_advance_rounding(ctx);
 	enqueue_dirichlet(ctx);
 	for (int axis = 0; axis < ctx->mesh.dims; axis++)
 	{
@@ -245,8 +264,17 @@ int bfvm_initialize(bfvm_ctx* ctx, const bfvm_microenvironment_desc* desc)
 		ctx->compute_internalized = desc->compute_internalized_substrates != 0;
 
 		const size_t VS = m.V * S;
-		dalloc(ctx->dens, VS);
-		BFVM_HIP(hipMemsetAsync(ctx->dens, 0, sizeof(float) * VS, ctx->stream));
+		ctx->storage = ctx->storage_request;
+		dalloc_bytes(ctx->dens, VS * storage_bytes(ctx->storage));
+		BFVM_HIP(hipMemsetAsync(ctx->dens, 0, VS * storage_bytes(ctx->storage), ctx->stream));
+		ctx->h_scale.assign(S, 1.f);
+		dalloc(ctx->dscale, S);
+		dalloc(ctx->dinv, S);
+		dalloc(ctx->maxbits, S);
+		dalloc(ctx->rounding_step, 1);
+		BFVM_HIP(hipMemsetAsync(ctx->rounding_step, 0, sizeof(unsigned), ctx->stream));
+		BFVM_HIP(hipMemcpy(ctx->dscale, ctx->h_scale.data(), sizeof(float) * S, hipMemcpyHostToDevice));
+		BFVM_HIP(hipMemcpy(ctx->dinv, ctx->h_scale.data(), sizeof(float) * S, hipMemcpyHostToDevice));
 		ctx->grad_elems = VS * dims;
 		dalloc(ctx->grad, ctx->grad_elems);
 		BFVM_HIP(hipMemsetAsync(ctx->grad, 0, sizeof(float) * ctx->grad_elems, ctx->stream));
@@ -277,6 +305,11 @@ int bfvm_initialize(bfvm_ctx* ctx, const bfvm_microenvironment_desc* desc)
 				}
 			}
 		ctx->has_faces = any;
+		ctx->bound_max.assign(S, 0.f);
+		for (int f = 0; f < 6; f++)
+			for (int s = 0; s < S; s++)
+				if (fcond[f * S + s])
+					ctx->bound_max[s] = std::max(ctx->bound_max[s], std::fabs(fval[f * S + s]));
 		dalloc(ctx->face_val, 6 * S);
 		dalloc(ctx->face_cond, 6 * S);
 		BFVM_HIP(hipMemcpy(ctx->face_val, fval.data(), sizeof(float) * 6 * S, hipMemcpyHostToDevice));
@@ -324,6 +357,9 @@ int bfvm_initialize(bfvm_ctx* ctx, const bfvm_microenvironment_desc* desc)
 				}
 		}
 		ctx->n_interior = (int64_t)vox.size();
+		for (size_t k = 0; k < val.size(); k++)
+			if (cond[k])
+				ctx->bound_max[k % S] = std::max(ctx->bound_max[k % S], std::fabs(val[k]));
 		if (ctx->n_interior > 0)
 		{
 			dalloc(ctx->int_vox, vox.size());
@@ -358,6 +394,18 @@ int bfvm_initialize(bfvm_ctx* ctx, const bfvm_microenvironment_desc* desc)
 		BFVM_HIP(hipStreamSynchronize(ctx->stream));
 		ctx->initialized = true;
 		ctx->epoch++;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_set_storage(bfvm_ctx* ctx, int32_t storage)
+{
+	return guarded("bfvm_set_storage", [&] {
+		if (!ctx)
+			throw std::invalid_argument("null context");
+		if (storage != BFVM_STORAGE_F32 && storage != BFVM_STORAGE_BF16 && storage != BFVM_STORAGE_F16)
+			throw std::invalid_argument("unknown storage form");
+		ctx->storage_request = storage;
 		return BFVM_OK;
 	});
 }
@@ -436,6 +484,43 @@ int bfvm_set_gradient_tuning(bfvm_ctx* ctx, int32_t workgroups)
 	});
 }
 
+// FP16 scale of substrate s: the power of two 2^k with (largest |value|) * 2^k in [2^10, 2^11); 1 when the
+// substrate is all zeros.  BF16 and FP32 keep 1.
+static void choose_scales(bfvm_ctx* ctx, const std::vector<float>& max_abs)
+{
+	const int S = ctx->mesh.S;
+	for (int s = 0; s < S; s++)
+	{
+		float scale = 1.f;
+		const float mx = std::max(max_abs[s], ctx->bound_max[s]);
+		if (ctx->storage == BFVM_STORAGE_F16 && mx > 0.f && std::isfinite(mx))
+		{
+			int e;
+			std::frexp(mx, &e); // mx = f * 2^e, f in [0.5, 1)
+			scale = std::ldexp(1.f, std::min(100, std::max(-100, 11 - e)));
+		}
+		ctx->h_scale[s] = scale;
+	}
+	std::vector<float> inv(S);
+	for (int s = 0; s < S; s++)
+		inv[s] = 1.f / ctx->h_scale[s];
+	BFVM_HIP(hipMemcpyAsync(ctx->dscale, ctx->h_scale.data(), sizeof(float) * S, hipMemcpyHostToDevice, ctx->stream));
+	BFVM_HIP(hipMemcpyAsync(ctx->dinv, inv.data(), sizeof(float) * S, hipMemcpyHostToDevice, ctx->stream));
+	BFVM_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+static constexpr size_t staging_chunk = (size_t)1 << 25; // FP32 elements per conversion pass (128 MiB)
+
+static void ensure_staging(bfvm_ctx* ctx)
+{
+	const size_t want = std::min(staging_chunk, ctx->mesh.V * (size_t)ctx->mesh.S);
+	if (ctx->staging_elems < want)
+	{
+		dalloc(ctx->staging, want);
+		ctx->staging_elems = want;
+	}
+}
+
 int bfvm_store_densities(bfvm_ctx* ctx, const float* densities)
 {
 	return guarded("bfvm_store_densities", [&] {
@@ -443,8 +528,31 @@ int bfvm_store_densities(bfvm_ctx* ctx, const float* densities)
 		if (!densities)
 			throw std::invalid_argument("null densities");
 		BFVM_HIP(hipSetDevice(ctx->device));
-		BFVM_HIP(hipMemcpyAsync(ctx->dens, densities, sizeof(float) * ctx->mesh.V * ctx->mesh.S,
-								hipMemcpyHostToDevice, ctx->stream));
+		const size_t n = ctx->mesh.V * ctx->mesh.S;
+		if (ctx->storage == BFVM_STORAGE_F32)
+		{
+			BFVM_HIP(hipMemcpyAsync(ctx->dens, densities, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+			BFVM_HIP(hipStreamSynchronize(ctx->stream));
+			return BFVM_OK;
+		}
+		const int S = ctx->mesh.S;
+		std::vector<float> mx(S, 0.f);
+		if (ctx->storage == BFVM_STORAGE_F16)
+			for (size_t i = 0; i < n; i++)
+			{
+				const float a = std::fabs(densities[i]);
+				if (a > mx[i % S] && std::isfinite(a))
+					mx[i % S] = a;
+			}
+		choose_scales(ctx, mx);
+		ensure_staging(ctx);
+		for (size_t off = 0; off < n; off += ctx->staging_elems)
+		{
+			const size_t len = std::min(ctx->staging_elems, n - off);
+			BFVM_HIP(hipMemcpyAsync(ctx->staging, densities + off, sizeof(float) * len, hipMemcpyHostToDevice,
+									ctx->stream));
+			launch_pack(ctx, ctx->staging, (int64_t)off, (int64_t)len);
+		}
 		BFVM_HIP(hipStreamSynchronize(ctx->stream));
 		return BFVM_OK;
 	});
@@ -457,8 +565,28 @@ int bfvm_store_densities_device(bfvm_ctx* ctx, const float* device_densities)
 		if (!device_densities)
 			throw std::invalid_argument("null densities");
 		BFVM_HIP(hipSetDevice(ctx->device));
-		BFVM_HIP(hipMemcpyAsync(ctx->dens, device_densities, sizeof(float) * ctx->mesh.V * ctx->mesh.S,
-								hipMemcpyDeviceToDevice, ctx->stream));
+		const size_t n = ctx->mesh.V * ctx->mesh.S;
+		if (ctx->storage == BFVM_STORAGE_F32)
+		{
+			BFVM_HIP(hipMemcpyAsync(ctx->dens, device_densities, sizeof(float) * n, hipMemcpyDeviceToDevice,
+									ctx->stream));
+			return BFVM_OK;
+		}
+		const int S = ctx->mesh.S;
+		std::vector<float> mx(S, 0.f);
+		if (ctx->storage == BFVM_STORAGE_F16)
+		{
+			BFVM_HIP(hipMemsetAsync(ctx->maxbits, 0, sizeof(unsigned) * S, ctx->stream));
+			launch_max_abs(ctx, device_densities, (int64_t)n, ctx->maxbits);
+			std::vector<unsigned> bits(S);
+			BFVM_HIP(hipMemcpyAsync(bits.data(), ctx->maxbits, sizeof(unsigned) * S, hipMemcpyDeviceToHost,
+									ctx->stream));
+			BFVM_HIP(hipStreamSynchronize(ctx->stream));
+			for (int s = 0; s < S; s++)
+				std::memcpy(&mx[s], &bits[s], sizeof(float));
+		}
+		choose_scales(ctx, mx);
+		launch_pack(ctx, device_densities, 0, (int64_t)n);
 		return BFVM_OK;
 	});
 }
@@ -529,8 +657,21 @@ int bfvm_load_densities(bfvm_ctx* ctx, float* densities)
 		if (!densities)
 			throw std::invalid_argument("null densities");
 		BFVM_HIP(hipSetDevice(ctx->device));
-		BFVM_HIP(hipMemcpyAsync(densities, ctx->dens, sizeof(float) * ctx->mesh.V * ctx->mesh.S,
-								hipMemcpyDeviceToHost, ctx->stream));
+		const size_t n = ctx->mesh.V * ctx->mesh.S;
+		if (ctx->storage == BFVM_STORAGE_F32)
+		{
+			BFVM_HIP(hipMemcpyAsync(densities, ctx->dens, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+			BFVM_HIP(hipStreamSynchronize(ctx->stream));
+			return BFVM_OK;
+		}
+		ensure_staging(ctx);
+		for (size_t off = 0; off < n; off += ctx->staging_elems)
+		{
+			const size_t len = std::min(ctx->staging_elems, n - off);
+			launch_unpack(ctx, ctx->staging, (int64_t)off, (int64_t)len);
+			BFVM_HIP(hipMemcpyAsync(densities + off, ctx->staging, sizeof(float) * len, hipMemcpyDeviceToHost,
+									ctx->stream));
+		}
 		BFVM_HIP(hipStreamSynchronize(ctx->stream));
 		return BFVM_OK;
 	});
@@ -688,6 +829,8 @@ int bfvm_get_device_buffers(bfvm_ctx* ctx, bfvm_device_buffers* out)
 			out->grid_shape[d] = ctx->mesh.grid[d];
 		out->substrates_count = ctx->mesh.S;
 		out->stream = (void*)ctx->stream;
+		out->storage = ctx->storage;
+		out->density_scales = ctx->dscale;
 		return BFVM_OK;
 	});
 }

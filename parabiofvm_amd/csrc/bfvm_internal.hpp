@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "parabiofvm_hip.h"
@@ -46,6 +47,129 @@ __device__ __forceinline__ void set_comp(T& v, int c, float x)
 {
 	v[c] = x;
 }
+
+// Density storage in HBM: FP32, or BF16 / FP16 holding value * 2^k_s (a per-substrate power-of-two scale).
+// Arithmetic is always FP32 in registers.  Conversions to 16 bits round to nearest even (dstore) for values
+// that are set (uploads, Dirichlet) and stochastically (dstore_sr) for values that accumulate updates.
+enum storage_kind
+{
+	store_f32 = BFVM_STORAGE_F32,
+	store_bf16 = BFVM_STORAGE_BF16,
+	store_f16 = BFVM_STORAGE_F16
+};
+
+template <int K>
+struct store_of
+{
+	using elem = float;
+};
+template <>
+struct store_of<store_bf16>
+{
+	using elem = __bf16;
+};
+template <>
+struct store_of<store_f16>
+{
+	using elem = _Float16;
+};
+
+template <int K, int V>
+struct svec_of
+{
+	using type = typename store_of<K>::elem __attribute__((ext_vector_type(V)));
+};
+template <int K>
+struct svec_of<K, 1>
+{
+	using type = typename store_of<K>::elem;
+};
+
+// V consecutive stored densities starting at element `idx`, as FP32 (still scaled)
+template <int K, int V>
+__device__ __forceinline__ typename vec_of<V>::type dload(const void* base, int64_t idx)
+{
+	using st = typename svec_of<K, V>::type;
+	const st v = *(const st*)((const typename store_of<K>::elem*)base + idx);
+	if constexpr (K == store_f32)
+		return v;
+	else if constexpr (V == 1)
+		return (float)v;
+	else
+		return __builtin_convertvector(v, typename vec_of<V>::type);
+}
+
+template <int K, int V>
+__device__ __forceinline__ void dstore(void* base, int64_t idx, typename vec_of<V>::type x)
+{
+	using st = typename svec_of<K, V>::type;
+	st* p = (st*)((typename store_of<K>::elem*)base + idx);
+	if constexpr (K == store_f32)
+		*p = x;
+	else if constexpr (V == 1)
+		*p = (typename store_of<K>::elem)x;
+	else
+		*p = __builtin_convertvector(x, st);
+}
+
+// 32-bit integer hash (xorshift-multiply avalanche, "lowbias32" constants of C. Wellons' hash-prospector).
+__host__ __device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+	x ^= x >> 16;
+	x *= 0x7feb352du;
+	x ^= x >> 15;
+	x *= 0x846ca68bu;
+	x ^= x >> 16;
+	return x;
+}
+
+// Per-launch key of the stochastic rounding: the device step counter and a salt naming the store site.
+__device__ __forceinline__ uint32_t rounding_key(const unsigned* counter, uint32_t salt)
+{
+	return mix32(*counter * 16u + salt);
+}
+
+// Store of values that accumulate small updates step after step (sweeps, secretion): 16-bit forms round
+// stochastically — a uniform random integer below one unit of the dropped bits is added to the FP32 bit
+// pattern and the dropped bits are cleared, so the rounding is unbiased and an update smaller than half a
+// 16-bit ulp is kept in expectation instead of being lost every step.  The random bits hash the element
+// index with `key`.  BF16 drops 16 bits; FP16 (values pre-scaled into its normal range) drops 13.  The
+// truncated value is exactly representable, so the final conversion is exact.  FP32 stores unchanged.
+template <int K, int V>
+__device__ __forceinline__ void dstore_sr(void* base, int64_t idx, typename vec_of<V>::type x, uint32_t key)
+{
+	if constexpr (K == store_f32)
+		dstore<K, V>(base, idx, x);
+	else
+	{
+		constexpr uint32_t drop = K == store_bf16 ? 0xFFFFu : 0x1FFFu;
+		const uint32_t h = (uint32_t)idx ^ ((uint32_t)((uint64_t)idx >> 32) * 0x9E3779B9u) ^ key;
+		typename vec_of<V>::type y;
+#pragma unroll
+		for (int c = 0; c < V; c++)
+		{
+			const uint32_t r = mix32(h + (uint32_t)c);
+			const uint32_t b = (__float_as_uint(comp(x, c)) + (r & drop)) & ~drop;
+			set_comp(y, c, __uint_as_float(b));
+		}
+		dstore<K, V>(base, idx, y);
+	}
+}
+
+// Calls f(std::integral_constant<int, K>) for the storage form K of a context.
+template <typename F>
+inline void by_storage(int storage, F&& f)
+{
+	if (storage == BFVM_STORAGE_BF16)
+		f(std::integral_constant<int, store_bf16>());
+	else if (storage == BFVM_STORAGE_F16)
+		f(std::integral_constant<int, store_f16>());
+	else
+		f(std::integral_constant<int, store_f32>());
+}
+
+inline size_t storage_bytes(int storage) { return storage == BFVM_STORAGE_F32 ? 4 : 2; }
+
 
 void set_error(const std::string& msg);
 
@@ -139,7 +263,18 @@ struct bfvm_ctx
 	float* d_decay = nullptr;
 	bool compute_internalized = false;
 
-	float* dens = nullptr;
+	// densities [V*S] in the storage form (float, __bf16 or _Float16); 16-bit forms hold value * dscale[s]
+	void* dens = nullptr;
+	int storage = BFVM_STORAGE_F32;		 // form of `dens`
+	int storage_request = BFVM_STORAGE_F32; // form the next bfvm_initialize allocates
+	float* dscale = nullptr;				 // [S] powers of two (1 for FP32)
+	float* dinv = nullptr;					 // [S] 1 / dscale
+	std::vector<float> h_scale;
+	std::vector<float> bound_max; // [S] largest |Dirichlet value| with its condition set
+	unsigned* maxbits = nullptr;  // [S] device scratch of the FP16 scale choice
+	unsigned* rounding_step = nullptr; // device counter keying the stochastic rounding of 16-bit stores
+	float* staging = nullptr;	  // FP32 staging for host <-> 16-bit conversions
+	size_t staging_elems = 0;
 	float* grad = nullptr;
 	size_t grad_elems = 0;
 
@@ -180,6 +315,11 @@ void launch_dirichlet_interior(bfvm_ctx* ctx);
 
 void launch_gradient(bfvm_ctx* ctx);
 void launch_copy(bfvm_ctx* ctx);
+// FP32 <-> storage conversion of n densities starting at element `offset` (with the per-substrate scale)
+void launch_pack(bfvm_ctx* ctx, const float* src, int64_t offset, int64_t n);
+void launch_unpack(bfvm_ctx* ctx, float* dst, int64_t offset, int64_t n);
+void launch_max_abs(bfvm_ctx* ctx, const float* src, int64_t n, unsigned* out);
+void launch_advance_rounding(bfvm_ctx* ctx);
 
 void launch_cell(bfvm_ctx* ctx, bool recompute);
 void launch_cell_voxels(bfvm_ctx* ctx);

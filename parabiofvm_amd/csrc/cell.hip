@@ -136,8 +136,11 @@ __global__ __launch_bounds__(256) void cell_sum_kernel(int A, int S, const int* 
 // Apply, part 1 (host/cell_solver.cpp:137-152, 173-195, 208-234).  One thread per (agent, substrate).
 // No conflict and internalized: the fused update of I and D.  Conflict and internalized: I with the
 // agent's own terms and the current D (D is updated by part 2 after the kernel boundary).
+template <int K>
 __global__ __launch_bounds__(256) void cell_apply_kernel(int A, int S, int with_internalized, int voxel_volume,
-														 const int64_t* __restrict__ vox, float* __restrict__ dens,
+														 const int64_t* __restrict__ vox, void* __restrict__ dens,
+														 const float* __restrict__ scale, const float* __restrict__ inv,
+														 const unsigned* __restrict__ rounding,
 														 float* __restrict__ intern, const float* __restrict__ num,
 														 const float* __restrict__ den, const float* __restrict__ fac,
 														 const float* __restrict__ rnum,
@@ -155,26 +158,28 @@ __global__ __launch_bounds__(256) void cell_apply_kernel(int A, int S, int with_
 	const int64_t v = vox[i];
 	if (v < 0)
 		return;
-	float* D = dens + v * S + s;
+	const int64_t at = v * S + s;
 	const float vv = (float)voxel_volume;
+	const float d = K == store_f32 ? dload<K, 1>(dens, at) : dload<K, 1>(dens, at) * inv[s];
 	if (*conflict == 0)
 	{
-		const float d = *D;
 		const float rn = rnum[k], rd = rden[k], rf = rfac[k];
 		intern[k] = intern[k] - vv * (((1 - rd) * d + rn) / (rd) + rf);
-		*D = (d + rn) / rd + rf;
+		const float nd = (d + rn) / rd + rf;
+		dstore_sr<K, 1>(dens, at, K == store_f32 ? nd : nd * scale[s], K == store_f32 ? 0u : rounding_key(rounding, 5));
 	}
 	else
-	{
-		const float d = *D;
 		intern[k] = intern[k] - vv * (((-den[k]) * d + num[k]) / (1 + den[k]) + fac[k]);
-	}
 }
 
 // Apply, part 2 (host/cell_solver.cpp:154-171, 236-244): the ballot winner of each voxel updates D with
 // the reduced sums.  Skipped when part 1 already did the fused update.
+template <int K>
 __global__ __launch_bounds__(256) void cell_densities_kernel(int A, int S, int with_internalized,
-															 const int64_t* __restrict__ vox, float* __restrict__ dens,
+															 const int64_t* __restrict__ vox, void* __restrict__ dens,
+															 const float* __restrict__ scale,
+															 const float* __restrict__ inv,
+															 const unsigned* __restrict__ rounding,
 															 const float* __restrict__ rnum,
 															 const float* __restrict__ rden,
 															 const float* __restrict__ rfac,
@@ -193,13 +198,18 @@ __global__ __launch_bounds__(256) void cell_densities_kernel(int A, int S, int w
 		return;
 	if (ballots[v] != (int)i)
 		return;
-	float* D = dens + v * S + s;
-	*D = (*D + rnum[k]) / rden[k] + rfac[k];
+	const int64_t at = v * S + s;
+	const float d = K == store_f32 ? dload<K, 1>(dens, at) : dload<K, 1>(dens, at) * inv[s];
+	const float nd = (d + rnum[k]) / rden[k] + rfac[k];
+	dstore_sr<K, 1>(dens, at, K == store_f32 ? nd : nd * scale[s], K == store_f32 ? 0u : rounding_key(rounding, 6));
 }
 
 // host/cell_solver.cpp:301-328: D += I*F/vv, I = 0 for one agent.
+template <int K>
 __global__ void cell_release_kernel(int S, int64_t i, float voxel_volume, const int64_t* __restrict__ vox,
-									float* __restrict__ dens, float* __restrict__ intern,
+									void* __restrict__ dens, const float* __restrict__ scale,
+									const float* __restrict__ inv, const unsigned* __restrict__ rounding,
+									float* __restrict__ intern,
 									const float* __restrict__ frac)
 {
 	const int s = threadIdx.x + blockIdx.x * blockDim.x;
@@ -209,7 +219,10 @@ __global__ void cell_release_kernel(int S, int64_t i, float voxel_volume, const 
 	if (v < 0)
 		return;
 	const int64_t k = i * S + s;
-	dens[v * S + s] = dens[v * S + s] + intern[k] * frac[k] / voxel_volume;
+	const int64_t at = v * S + s;
+	const float d = K == store_f32 ? dload<K, 1>(dens, at) : dload<K, 1>(dens, at) * inv[s];
+	const float nd = d + intern[k] * frac[k] / voxel_volume;
+	dstore_sr<K, 1>(dens, at, K == store_f32 ? nd : nd * scale[s], K == store_f32 ? 0u : rounding_key(rounding, 7));
 	intern[k] = 0;
 }
 
@@ -283,22 +296,29 @@ void launch_cell(bfvm_ctx* ctx, bool recompute)
 	}
 	if (A == 0)
 		return;
-	hipLaunchKernelGGL(cell_apply_kernel, dim3(pair_blocks), dim3(256), 0, ctx->stream, A, S,
-					   (int)ctx->compute_internalized, voxel_volume(ctx), a.vox, ctx->dens, a.intern, a.num, a.den,
-					   a.fac, a.rnum, a.rden, a.rfac, ctx->conflict);
-	BFVM_HIP(hipGetLastError());
-	hipLaunchKernelGGL(cell_densities_kernel, dim3(pair_blocks), dim3(256), 0, ctx->stream, A, S,
-					   (int)ctx->compute_internalized, a.vox, ctx->dens, a.rnum, a.rden, a.rfac, ctx->ballots,
-					   ctx->conflict);
-	BFVM_HIP(hipGetLastError());
+	by_storage(ctx->storage, [&](auto kind) {
+		constexpr int K = decltype(kind)::value;
+		hipLaunchKernelGGL(cell_apply_kernel<K>, dim3(pair_blocks), dim3(256), 0, ctx->stream, A, S,
+						   (int)ctx->compute_internalized, voxel_volume(ctx), a.vox, ctx->dens, ctx->dscale, ctx->dinv,
+						   ctx->rounding_step, a.intern, a.num, a.den, a.fac, a.rnum, a.rden, a.rfac, ctx->conflict);
+		BFVM_HIP(hipGetLastError());
+		hipLaunchKernelGGL(cell_densities_kernel<K>, dim3(pair_blocks), dim3(256), 0, ctx->stream, A, S,
+						   (int)ctx->compute_internalized, a.vox, ctx->dens, ctx->dscale, ctx->dinv, ctx->rounding_step,
+						   a.rnum, a.rden,
+						   a.rfac, ctx->ballots, ctx->conflict);
+		BFVM_HIP(hipGetLastError());
+	});
 }
 
 void launch_cell_release(bfvm_ctx* ctx, int index)
 {
 	const auto& a = ctx->ag;
 	const int S = ctx->mesh.S;
-	hipLaunchKernelGGL(cell_release_kernel, dim3((S + 63) / 64), dim3(64), 0, ctx->stream, S, (int64_t)index,
-					   (float)voxel_volume(ctx), a.vox, ctx->dens, a.intern, a.frac);
+	by_storage(ctx->storage, [&](auto kind) {
+		hipLaunchKernelGGL(cell_release_kernel<decltype(kind)::value>, dim3((S + 63) / 64), dim3(64), 0, ctx->stream,
+						   S, (int64_t)index, (float)voxel_volume(ctx), a.vox, ctx->dens, ctx->dscale, ctx->dinv,
+						   ctx->rounding_step, a.intern, a.frac);
+	});
 	BFVM_HIP(hipGetLastError());
 }
 

@@ -272,7 +272,10 @@ struct face_epilogue
 
 struct sweep_geometry
 {
-	float* data;
+	void* data;		   // densities in the storage form
+	const float* scale; // [S] storage scale (16-bit forms)
+	const float* inv;	// [S] 1 / scale
+	const unsigned* rounding; // step counter keying the stochastic rounding (16-bit forms)
 	int X, Y, Z, S, dims;
 	const float* fval;
 	const uint8_t* fcond;
@@ -343,7 +346,7 @@ __global__ __launch_bounds__(256) void sweep_exact_kernel(sweep_geometry g, int6
 	int coord[3];
 	line_coords<AXIS>(g, line, base, s, coord);
 	const int64_t stride = line_stride<AXIS>(g);
-	float* p = g.data + base;
+	float* p = (float*)g.data + base; // FP32 storage only: the forward pass keeps its intermediates in memory
 	const int grid[3] = { g.X, g.Y, g.Z };
 	face_epilogue fe;
 	fe.init(g.fval, g.fcond, g.S, s, AXIS, coord, grid, g.dims);
@@ -369,6 +372,7 @@ __global__ __launch_bounds__(256) void sweep_exact_kernel(sweep_geometry g, int6
 }
 
 // Axis of length one: the sweep is x *= b (b = 1/(1 + decay*dt/dims)), then the faces.
+template <int K>
 __global__ __launch_bounds__(256) void sweep_single_kernel(sweep_geometry g, int axis, const float* __restrict__ b)
 {
 	const int64_t total = (int64_t)g.X * g.Y * g.Z * g.S;
@@ -381,7 +385,13 @@ __global__ __launch_bounds__(256) void sweep_single_kernel(sweep_geometry g, int
 	const int grid[3] = { g.X, g.Y, g.Z };
 	face_epilogue fe;
 	fe.init(g.fval, g.fcond, g.S, s, axis, coord, grid, g.dims);
-	g.data[idx] = fe.apply(g.data[idx] * b[s], 0, 1);
+	if constexpr (K == store_f32)
+		((float*)g.data)[idx] = fe.apply(((float*)g.data)[idx] * b[s], 0, 1);
+	else
+	{
+		const float d = dload<K, 1>(g.data, idx) * g.inv[s];
+		dstore_sr<K, 1>(g.data, idx, fe.apply(d * b[s], 0, 1) * g.scale[s], rounding_key(g.rounding, 1 + axis));
+	}
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -495,7 +505,7 @@ __device__ __forceinline__ typename vec_of<V>::type faces_slow(typename vec_of<V
 // (a block of 16 float4 values plus coefficients does not fit the 128 of a 1024-thread workgroup).
 constexpr int max_group_threads(int V) { return V == 4 ? 512 : 1024; }
 
-template <int M, int AXIS, int V>
+template <int M, int AXIS, int V, int K>
 __global__ __launch_bounds__(max_group_threads(V)) void sweep_partitioned_kernel(part_args a)
 {
 	using vec = typename vec_of<V>::type;
@@ -521,6 +531,7 @@ __global__ __launch_bounds__(max_group_threads(V)) void sweep_partitioned_kernel
 	const int grid[3] = { g.X, g.Y, g.Z };
 	const size_t fstride = (size_t)a.ntypes * a.mb * S;
 	const size_t rstride = (size_t)2 * P * S;
+	const uint32_t key = K == store_f32 ? 0u : rounding_key(g.rounding, 1 + AXIS);
 
 	int start = 0, mk = 0, type = 0;
 	if (k >= 0)
@@ -540,7 +551,7 @@ __global__ __launch_bounds__(max_group_threads(V)) void sweep_partitioned_kernel
 		int coord[3] = { 0, 0, 0 };
 		if (active)
 			vline_coords<AXIS, V>(g, line, base, s0, coord);
-		float* p = g.data + base + (int64_t)start * stride;
+		const int64_t off0 = base + (int64_t)start * stride; // element offset of the block's first value
 		const float* cr = lcoef + 0 * fstride + (size_t)type * a.mb * S + s0;
 		const float* cg = lcoef + 1 * fstride + (size_t)type * a.mb * S + s0;
 		const float* ca = lcoef + 2 * fstride + (size_t)type * a.mb * S + s0;
@@ -552,7 +563,14 @@ __global__ __launch_bounds__(max_group_threads(V)) void sweep_partitioned_kernel
 #pragma unroll
 			for (int j = 0; j < M; j++)
 				if (j < mk)
-					v[j] = vload<V>(p + j * stride);
+					v[j] = dload<K, V>(g.data, off0 + j * stride);
+			if constexpr (K != store_f32)
+			{
+				const vec inv = vload<V>(g.inv + s0);
+#pragma unroll
+				for (int j = 0; j < M; j++)
+					v[j] = v[j] * inv;
+			}
 			// forward elimination inside the block
 			const vec As = vload<V>(a.A + s0);
 #pragma unroll
@@ -659,7 +677,9 @@ __global__ __launch_bounds__(max_group_threads(V)) void sweep_partitioned_kernel
 					const int i = start + j;
 					if (line_face || (i == 0 && end_min) || (i == a.n - 1 && end_max))
 						x = faces_slow<AXIS, V>(x, g, s0, coord, i);
-					vstore<V>(p + j * stride, x);
+					if constexpr (K != store_f32)
+						x = x * vload<V>(g.scale + s0);
+					dstore_sr<K, V>(g.data, off0 + j * stride, x, key);
 				}
 			}
 		}
@@ -685,6 +705,9 @@ static sweep_geometry geometry(bfvm_ctx* ctx)
 {
 	sweep_geometry g;
 	g.data = ctx->dens;
+	g.scale = ctx->dscale;
+	g.inv = ctx->dinv;
+	g.rounding = ctx->rounding_step;
 	g.X = ctx->mesh.grid[0];
 	g.Y = ctx->mesh.grid[1];
 	g.Z = ctx->mesh.grid[2];
@@ -755,9 +778,18 @@ void build_partition(bfvm_ctx* ctx, int axis, int chunk, int lines_request)
 		return;
 	// 4 substrates per thread when they pack into 16-byte vectors (register chunk 16: 64 data VGPRs)
 	const bool vec4 = S % 4 == 0;
-	int M = supported_chunk(chunk > 0 ? chunk : (vec4 ? 16 : 32));
-	for (;; M = supported_chunk(M + 1))
+	// candidate register chunks, smallest first from the request; 16-bit storage instantiates 16 and 32 only
+	const bool wide = ctx->storage != BFVM_STORAGE_F32;
+	std::vector<int> candidates;
+	for (int c : chunk_sizes)
+		if (c >= supported_chunk(chunk > 0 ? chunk : (vec4 ? 16 : 32)) && (!wide || c == 16 || c == 32))
+			candidates.push_back(c);
+	if (candidates.empty())
+		candidates.push_back(32);
+	for (size_t ci = 0;; ci++)
 	{
+		const int M = candidates[ci];
+		const bool last_candidate = ci + 1 == candidates.size();
 		partition p;
 		p.n = n;
 		p.chunk = M;
@@ -774,7 +806,7 @@ void build_partition(bfvm_ctx* ctx, int axis, int chunk, int lines_request)
 		p.threads = threads;
 		const bool fits = threads <= max_threads && lpg * p.P <= threads &&
 						  partition_lds_bytes(p, S, lpg) <= 64 * 1024 && (p.P == 1 || p.base >= 3);
-		if (fits || M == 64)
+		if (fits || last_candidate)
 		{
 			if (!fits)
 				throw std::runtime_error("partitioned sweep: axis too long for the register/LDS budget "
@@ -802,7 +834,7 @@ void launch_partition_coefficients(bfvm_ctx* ctx, int axis)
 	BFVM_HIP(hipGetLastError());
 }
 
-template <int M, int AXIS, int V>
+template <int M, int AXIS, int V, int K>
 static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const axis_tables& t)
 {
 	const partition& p = t.part;
@@ -825,35 +857,47 @@ static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const a
 	const size_t lds = partition_lds_bytes(p, g.S, lpg);
 	const int64_t max_blocks = 8192;
 	const int blocks = (int)std::min<int64_t>(a.groups, max_blocks);
-	hipLaunchKernelGGL((sweep_partitioned_kernel<M, AXIS, V>), dim3(blocks), dim3(threads), lds, ctx->stream, a);
+	hipLaunchKernelGGL((sweep_partitioned_kernel<M, AXIS, V, K>), dim3(blocks), dim3(threads), lds, ctx->stream, a);
 	BFVM_HIP(hipGetLastError());
 }
 
-template <int AXIS>
+// FP32 storage: every register chunk; 16-bit storage: chunks 16 and 32 (build_partition restricts them)
+template <int AXIS, int K>
 static void launch_partitioned_axis(bfvm_ctx* ctx, const sweep_geometry& g, const axis_tables& t)
 {
 	if (t.part.vec == 4)
 	{
-		launch_partitioned_t<16, AXIS, 4>(ctx, g, t);
+		launch_partitioned_t<16, AXIS, 4, K>(ctx, g, t);
 		return;
 	}
-	switch (t.part.chunk)
+	if constexpr (K != store_f32)
 	{
-	case 8:
-		launch_partitioned_t<8, AXIS, 1>(ctx, g, t);
-		break;
-	case 16:
-		launch_partitioned_t<16, AXIS, 1>(ctx, g, t);
-		break;
-	case 32:
-		launch_partitioned_t<32, AXIS, 1>(ctx, g, t);
-		break;
-	case 48:
-		launch_partitioned_t<48, AXIS, 1>(ctx, g, t);
-		break;
-	default:
-		launch_partitioned_t<64, AXIS, 1>(ctx, g, t);
-		break;
+		if (t.part.chunk == 16)
+			launch_partitioned_t<16, AXIS, 1, K>(ctx, g, t);
+		else
+			launch_partitioned_t<32, AXIS, 1, K>(ctx, g, t);
+		return;
+	}
+	else
+	{
+		switch (t.part.chunk)
+		{
+		case 8:
+			launch_partitioned_t<8, AXIS, 1, K>(ctx, g, t);
+			break;
+		case 16:
+			launch_partitioned_t<16, AXIS, 1, K>(ctx, g, t);
+			break;
+		case 32:
+			launch_partitioned_t<32, AXIS, 1, K>(ctx, g, t);
+			break;
+		case 48:
+			launch_partitioned_t<48, AXIS, 1, K>(ctx, g, t);
+			break;
+		default:
+			launch_partitioned_t<64, AXIS, 1, K>(ctx, g, t);
+			break;
+		}
 	}
 }
 
@@ -869,7 +913,8 @@ static void launch_exact_axis(bfvm_ctx* ctx, const sweep_geometry& g, const axis
 	BFVM_HIP(hipGetLastError());
 }
 
-void launch_sweep(bfvm_ctx* ctx, int axis)
+template <int K>
+static void launch_sweep_storage(bfvm_ctx* ctx, int axis)
 {
 	const sweep_geometry g = geometry(ctx);
 	const axis_tables& t = ctx->axes[axis];
@@ -877,14 +922,16 @@ void launch_sweep(bfvm_ctx* ctx, int axis)
 	if (n == 1)
 	{
 		const int64_t total = (int64_t)g.X * g.Y * g.Z * g.S;
-		hipLaunchKernelGGL(sweep_single_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, g,
-						   axis, t.exact_b);
+		hipLaunchKernelGGL((sweep_single_kernel<K>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream,
+						   g, axis, t.exact_b);
 		BFVM_HIP(hipGetLastError());
 		return;
 	}
 	if (ctx->algorithm == BFVM_SWEEP_EXACT)
 	{
-		if (axis == 0)
+		if constexpr (K != store_f32)
+			throw std::runtime_error("the exact sweep needs FP32 density storage");
+		else if (axis == 0)
 			launch_exact_axis<0>(ctx, g, t);
 		else if (axis == 1)
 			launch_exact_axis<1>(ctx, g, t);
@@ -893,11 +940,27 @@ void launch_sweep(bfvm_ctx* ctx, int axis)
 		return;
 	}
 	if (axis == 0)
-		launch_partitioned_axis<0>(ctx, g, t);
+		launch_partitioned_axis<0, K>(ctx, g, t);
 	else if (axis == 1)
-		launch_partitioned_axis<1>(ctx, g, t);
+		launch_partitioned_axis<1, K>(ctx, g, t);
 	else
-		launch_partitioned_axis<2>(ctx, g, t);
+		launch_partitioned_axis<2, K>(ctx, g, t);
+}
+
+void launch_sweep(bfvm_ctx* ctx, int axis)
+{
+	switch (ctx->storage)
+	{
+	case BFVM_STORAGE_BF16:
+		launch_sweep_storage<store_bf16>(ctx, axis);
+		break;
+	case BFVM_STORAGE_F16:
+		launch_sweep_storage<store_f16>(ctx, axis);
+		break;
+	default:
+		launch_sweep_storage<store_f32>(ctx, axis);
+		break;
+	}
 }
 
 } // namespace bfvm

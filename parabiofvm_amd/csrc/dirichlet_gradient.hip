@@ -14,8 +14,10 @@ namespace bfvm {
 // so threads that reach the same edge voxel from different faces write the same value.
 // Face voxels are enumerated as the two x planes (2*Y*Z), then the two y planes (2*X*Z), then the two z
 // planes (2*X*Y) of the axes the mesh has.
-__global__ __launch_bounds__(256) void dirichlet_faces_kernel(float* __restrict__ dens, const float* __restrict__ fval,
-															  const uint8_t* __restrict__ fcond, int X, int Y, int Z,
+template <int K>
+__global__ __launch_bounds__(256) void dirichlet_faces_kernel(void* __restrict__ dens, const float* __restrict__ fval,
+															  const uint8_t* __restrict__ fcond,
+															  const float* __restrict__ scale, int X, int Y, int Z,
 															  int S, int dims, int64_t total)
 {
 	const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -70,15 +72,17 @@ __global__ __launch_bounds__(256) void dirichlet_faces_kernel(float* __restrict_
 		}
 	}
 	if (set)
-		dens[(((int64_t)z * Y + y) * X + x) * S + s] = v;
+		dstore<K, 1>(dens, (((int64_t)z * Y + y) * X + x) * S + s, K == store_f32 ? v : v * scale[s]);
 }
 
 // Interior list, de-duplicated per voxel at upload (host/dirichlet_solver.cpp:22-44).  One thread per
 // (entry, substrate).
-__global__ __launch_bounds__(256) void dirichlet_interior_kernel(float* __restrict__ dens,
+template <int K>
+__global__ __launch_bounds__(256) void dirichlet_interior_kernel(void* __restrict__ dens,
 																 const int64_t* __restrict__ vox,
 																 const float* __restrict__ val,
-																 const uint8_t* __restrict__ cond, int S,
+																 const uint8_t* __restrict__ cond,
+																 const float* __restrict__ scale, int S,
 																 int64_t total)
 {
 	const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -87,7 +91,7 @@ __global__ __launch_bounds__(256) void dirichlet_interior_kernel(float* __restri
 	const int64_t i = idx / S;
 	const int s = (int)(idx % S);
 	if (cond[idx])
-		dens[vox[i] * S + s] = val[idx];
+		dstore<K, 1>(dens, vox[i] * S + s, K == store_f32 ? val[idx] : val[idx] * scale[s]);
 }
 
 void launch_dirichlet_faces(bfvm_ctx* ctx)
@@ -102,8 +106,11 @@ void launch_dirichlet_faces(bfvm_ctx* ctx)
 	if (m.dims >= 3)
 		faces += 2 * (int64_t)X * Y;
 	const int64_t total = faces * m.S;
-	hipLaunchKernelGGL(dirichlet_faces_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream,
-					   ctx->dens, ctx->face_val, ctx->face_cond, X, Y, Z, m.S, m.dims, total);
+	by_storage(ctx->storage, [&](auto k) {
+		hipLaunchKernelGGL((dirichlet_faces_kernel<decltype(k)::value>), dim3((unsigned)((total + 255) / 256)),
+						   dim3(256), 0, ctx->stream, ctx->dens, ctx->face_val, ctx->face_cond, ctx->dscale, X, Y, Z,
+						   m.S, m.dims, total);
+	});
 	BFVM_HIP(hipGetLastError());
 }
 
@@ -112,8 +119,11 @@ void launch_dirichlet_interior(bfvm_ctx* ctx)
 	if (ctx->n_interior == 0)
 		return;
 	const int64_t total = ctx->n_interior * ctx->mesh.S;
-	hipLaunchKernelGGL(dirichlet_interior_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream,
-					   ctx->dens, ctx->int_vox, ctx->int_val, ctx->int_cond, ctx->mesh.S, total);
+	by_storage(ctx->storage, [&](auto k) {
+		hipLaunchKernelGGL((dirichlet_interior_kernel<decltype(k)::value>), dim3((unsigned)((total + 255) / 256)),
+						   dim3(256), 0, ctx->stream, ctx->dens, ctx->int_vox, ctx->int_val, ctx->int_cond, ctx->dscale,
+						   ctx->mesh.S, total);
+	});
 	BFVM_HIP(hipGetLastError());
 }
 
@@ -125,9 +135,11 @@ void launch_dirichlet_interior(bfvm_ctx* ctx)
 // (d[1] - d[0]) * (1.f/dx) and (d[n-1] - d[n-2]) * (1.f/dx), with the float reciprocals of the reference
 // (host/gradient_solver.cpp:17-18).  Output layout [d][z][y][x][s] in 3-D, [y][x][s][d] in 1-D/2-D
 // (src/traits.h:27-33, 55-62, 86-92).
-__global__ __launch_bounds__(256) void gradient_kernel(float* __restrict__ grad, const float* __restrict__ dens, int X,
-													   int Y, int Z, int S, int dims, float bx, float cx, float by,
-													   float cy, float bz, float cz, int64_t total)
+template <int K>
+__global__ __launch_bounds__(256) void gradient_kernel(float* __restrict__ grad, const void* __restrict__ dens,
+													   const float* __restrict__ inv, int X, int Y, int Z, int S,
+													   int dims, float bx, float cx, float by, float cy, float bz,
+													   float cz, int64_t total)
 {
 	const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (idx >= total)
@@ -138,32 +150,37 @@ __global__ __launch_bounds__(256) void gradient_kernel(float* __restrict__ grad,
 	const int z = (int)(v / ((int64_t)X * Y));
 	const int64_t sx = S, sy = (int64_t)X * S, sz = (int64_t)X * Y * S;
 
+	const int s = (int)(idx % S);
+	const float is = K == store_f32 ? 1.f : inv[s];
+	auto p = [&](int64_t o) {
+		const float d = dload<K, 1>(dens, idx + o);
+		return K == store_f32 ? d : d * is;
+	};
 	float g[3];
 	{
-		const float* p = dens + idx;
 		if (x == 0)
-			g[0] = (p[sx] - p[0]) * bx;
+			g[0] = (p(sx) - p(0)) * bx;
 		else if (x == X - 1)
-			g[0] = (p[0] - p[-sx]) * bx;
+			g[0] = (p(0) - p(-sx)) * bx;
 		else
-			g[0] = (p[sx] - p[-sx]) * cx;
+			g[0] = (p(sx) - p(-sx)) * cx;
 		if (dims >= 2)
 		{
 			if (y == 0)
-				g[1] = (p[sy] - p[0]) * by;
+				g[1] = (p(sy) - p(0)) * by;
 			else if (y == Y - 1)
-				g[1] = (p[0] - p[-sy]) * by;
+				g[1] = (p(0) - p(-sy)) * by;
 			else
-				g[1] = (p[sy] - p[-sy]) * cy;
+				g[1] = (p(sy) - p(-sy)) * cy;
 		}
 		if (dims >= 3)
 		{
 			if (z == 0)
-				g[2] = (p[sz] - p[0]) * bz;
+				g[2] = (p(sz) - p(0)) * bz;
 			else if (z == Z - 1)
-				g[2] = (p[0] - p[-sz]) * bz;
+				g[2] = (p(0) - p(-sz)) * bz;
 			else
-				g[2] = (p[sz] - p[-sz]) * cz;
+				g[2] = (p(sz) - p(-sz)) * cz;
 		}
 	}
 	if (dims == 3)
@@ -187,9 +204,10 @@ __global__ __launch_bounds__(256) void gradient_kernel(float* __restrict__ grad,
 constexpr int grad_tile = 256;
 constexpr int num_xcds = 8;
 
-template <int V>
-__global__ __launch_bounds__(grad_tile) void gradient3d_kernel(float* __restrict__ grad, const float* __restrict__ dens,
-															   int Y, int Z, int S, int XS, int tiles_per_row,
+template <int V, int K>
+__global__ __launch_bounds__(grad_tile) void gradient3d_kernel(float* __restrict__ grad, const void* __restrict__ dens,
+															   const float* __restrict__ inv, int Y, int Z, int S,
+															   int XS, int tiles_per_row,
 															   int zchunks, int zchunk, int ntiles, int per_xcd,
 															   float bx, float cx, float by, float cy, float bz,
 															   float cz, int64_t VS)
@@ -211,24 +229,32 @@ __global__ __launch_bounds__(grad_tile) void gradient3d_kernel(float* __restrict
 	const int z1 = min(Z, z0 + zchunk);
 	const int64_t plane = (int64_t)Y * XS;
 	const int64_t off = (int64_t)y * XS + q;
-	const float* col = dens + off;
 	float* g0 = grad + off;
 	float* g1 = g0 + VS;
 	float* g2 = g1 + VS;
 	const bool xlo = q < S, xhi = q >= XS - S;
 	const bool ylo = y == 0, yhi = y == Y - 1;
 
-	vec zm = z0 > 0 ? vload<V>(col + (int64_t)(z0 - 1) * plane) : vec(0.f);
-	vec c = vload<V>(col + (int64_t)z0 * plane);
+	vec is = vec(1.f);
+	if constexpr (K != store_f32)
+		is = vload<V>(inv + q % S);
+	auto ld = [&](int64_t o) {
+		const vec d = dload<K, V>(dens, off + o);
+		if constexpr (K == store_f32)
+			return d;
+		else
+			return d * is;
+	};
+	vec zm = z0 > 0 ? ld((int64_t)(z0 - 1) * plane) : vec(0.f);
+	vec c = ld((int64_t)z0 * plane);
 	for (int z = z0; z < z1; z++)
 	{
 		const int64_t o = (int64_t)z * plane;
-		const float* p = col + o;
-		const vec zp = z + 1 < Z ? vload<V>(p + plane) : vec(0.f);
-		const vec xp = xhi ? vec(0.f) : vload<V>(p + S);
-		const vec xm = xlo ? vec(0.f) : vload<V>(p - S);
-		const vec yp = yhi ? vec(0.f) : vload<V>(p + XS);
-		const vec ym = ylo ? vec(0.f) : vload<V>(p - XS);
+		const vec zp = z + 1 < Z ? ld(o + plane) : vec(0.f);
+		const vec xp = xhi ? vec(0.f) : ld(o + S);
+		const vec xm = xlo ? vec(0.f) : ld(o - S);
+		const vec yp = yhi ? vec(0.f) : ld(o + XS);
+		const vec ym = ylo ? vec(0.f) : ld(o - XS);
 		vec gx, gy, gz;
 		if (xlo)
 			gx = (xp - c) * bx;
@@ -268,8 +294,8 @@ __global__ __launch_bounds__(256) void copy_kernel(float4* __restrict__ dst, con
 
 void launch_copy(bfvm_ctx* ctx)
 {
-	const int64_t n = (int64_t)ctx->mesh.V * ctx->mesh.S;
-	const int64_t n4 = n / 4;
+	const int64_t bytes = (int64_t)ctx->mesh.V * ctx->mesh.S * (ctx->storage == BFVM_STORAGE_F32 ? 4 : 2);
+	const int64_t n4 = bytes / 16;
 	if (n4 == 0)
 		return;
 	hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, ctx->stream, (float4*)ctx->grad,
@@ -301,16 +327,96 @@ void launch_gradient(bfvm_ctx* ctx)
 		zchunks = (Z + zchunk - 1) / zchunk;
 		const int64_t ntiles = row_tiles * zchunks;
 		const int per_xcd = (int)((ntiles + num_xcds - 1) / num_xcds);
-		auto kernel = V == 4 ? gradient3d_kernel<4> : gradient3d_kernel<1>;
-		hipLaunchKernelGGL(kernel, dim3((unsigned)(per_xcd * num_xcds)), dim3(grad_tile), 0, ctx->stream, ctx->grad,
-						   ctx->dens, Y, Z, m.S, XS, tiles_per_row, zchunks, zchunk, (int)ntiles, per_xcd, b[0], c[0],
-						   b[1], c[1], b[2], c[2], total);
+		by_storage(ctx->storage, [&](auto k) {
+			constexpr int K = decltype(k)::value;
+			auto kernel = V == 4 ? gradient3d_kernel<4, K> : gradient3d_kernel<1, K>;
+			hipLaunchKernelGGL(kernel, dim3((unsigned)(per_xcd * num_xcds)), dim3(grad_tile), 0, ctx->stream,
+							   ctx->grad, ctx->dens, ctx->dinv, Y, Z, m.S, XS, tiles_per_row, zchunks, zchunk,
+							   (int)ntiles, per_xcd, b[0], c[0], b[1], c[1], b[2], c[2], total);
+		});
 		BFVM_HIP(hipGetLastError());
 		return;
 	}
-	hipLaunchKernelGGL(gradient_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, ctx->grad,
-					   ctx->dens, m.grid[0], m.grid[1], m.grid[2], m.S, m.dims, b[0], c[0], b[1], c[1], b[2], c[2],
-					   total);
+	by_storage(ctx->storage, [&](auto k) {
+		hipLaunchKernelGGL((gradient_kernel<decltype(k)::value>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+						   ctx->stream, ctx->grad, ctx->dens, ctx->dinv, m.grid[0], m.grid[1], m.grid[2], m.S, m.dims,
+						   b[0], c[0], b[1], c[1], b[2], c[2], total);
+	});
+	BFVM_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP32 <-> storage conversions (uploads and downloads of the densities)
+// ------------------------------------------------------------------------------------------------
+
+template <int K>
+__global__ __launch_bounds__(256) void pack_kernel(void* __restrict__ dens, const float* __restrict__ src,
+												   const float* __restrict__ scale, int S, int64_t offset, int64_t n)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const int s = (int)((offset + i) % S);
+	dstore<K, 1>(dens, offset + i, src[i] * scale[s]);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void unpack_kernel(float* __restrict__ dst, const void* __restrict__ dens,
+													 const float* __restrict__ inv, int S, int64_t offset, int64_t n)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const int s = (int)((offset + i) % S);
+	dst[i] = dload<K, 1>(dens, offset + i) * inv[s];
+}
+
+// per-substrate max |x| as the bit pattern of a non-negative float (which orders like an unsigned integer)
+__global__ __launch_bounds__(256) void max_abs_kernel(const float* __restrict__ src, int S, int64_t n,
+													  unsigned* __restrict__ out)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+	{
+		const float a = fabsf(src[i]);
+		if (a > 0.f && a <= 3.4e38f)
+			atomicMax(&out[i % S], __float_as_uint(a));
+	}
+}
+
+__global__ void advance_rounding_kernel(unsigned* counter)
+{
+	if (threadIdx.x == 0)
+		*counter = *counter + 1;
+}
+
+void launch_advance_rounding(bfvm_ctx* ctx)
+{
+	hipLaunchKernelGGL(advance_rounding_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->rounding_step);
+	BFVM_HIP(hipGetLastError());
+}
+
+void launch_pack(bfvm_ctx* ctx, const float* src, int64_t offset, int64_t n)
+{
+	by_storage(ctx->storage, [&](auto k) {
+		hipLaunchKernelGGL((pack_kernel<decltype(k)::value>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+						   ctx->stream, ctx->dens, src, ctx->dscale, ctx->mesh.S, offset, n);
+	});
+	BFVM_HIP(hipGetLastError());
+}
+
+void launch_unpack(bfvm_ctx* ctx, float* dst, int64_t offset, int64_t n)
+{
+	by_storage(ctx->storage, [&](auto k) {
+		hipLaunchKernelGGL((unpack_kernel<decltype(k)::value>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+						   ctx->stream, dst, ctx->dens, ctx->dinv, ctx->mesh.S, offset, n);
+	});
+	BFVM_HIP(hipGetLastError());
+}
+
+void launch_max_abs(bfvm_ctx* ctx, const float* src, int64_t n, unsigned* out)
+{
+	hipLaunchKernelGGL(max_abs_kernel, dim3(4096), dim3(256), 0, ctx->stream, src, ctx->mesh.S, n, out);
 	BFVM_HIP(hipGetLastError());
 }
 

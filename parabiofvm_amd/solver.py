@@ -104,6 +104,12 @@ class DeviceSolver:
     def set_compute_internalized(self, flag):
         self.compute_internalized = bool(flag)
 
+    def set_storage(self, storage):
+        """Density storage form in HBM ("f32", "bf16", "f16" or a BFVM_STORAGE_* value); applies at the next
+        initialize.  Host arrays stay float32 either way."""
+        code = _native.STORAGE[storage] if isinstance(storage, str) else int(storage)
+        check("bfvm_set_storage", self.lib.bfvm_set_storage(self.h, code))
+
     def set_sweep_algorithm(self, algorithm, chunk=0):
         self._algorithm = (int(algorithm), int(chunk))
         check("bfvm_set_sweep_algorithm", self.lib.bfvm_set_sweep_algorithm(self.h, *self._algorithm))
@@ -145,6 +151,11 @@ class DeviceSolver:
         """store_data_to_solver: densities and agent data to the device."""
         check("bfvm_store_densities", self.lib.bfvm_store_densities(self.h, self.densities.ctypes.data))
         self.store_agents()
+
+    def store_densities_device(self, device_ptr):
+        """Densities from device memory (a float32 [V*S] buffer on this context's device), enqueued on the
+        context's stream."""
+        check("bfvm_store_densities_device", self.lib.bfvm_store_densities_device(self.h, ctypes.c_void_p(device_ptr)))
 
     def store_agents(self):
         a = self._agents

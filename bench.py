@@ -31,8 +31,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(cfg):
-    """Bytes each kernel must move per launch (what a fused, single-pass design reads and writes once)."""
+def algorithmic_bytes(cfg, e=4):
+    """Bytes each kernel must move per launch (what a fused, single-pass design reads and writes once), with
+    densities stored in `e` bytes (4 for FP32, 2 for BF16/FP16); gradients are always FP32."""
     X, Y, Z = cfg["grid"]
     S = cfg["S"]
     VS = X * Y * Z * S
@@ -40,17 +41,18 @@ def algorithmic_bytes(cfg):
     n_int = 0 if cfg["interior"] is None else cfg["interior"][0].size // 3
     face_elems = 2 * (X * Y + Y * Z + X * Z) * S if cfg["faces"] else 0
     per = {
-        "sweep_x": 8 * VS, "sweep_y": 8 * VS, "sweep_z": 8 * VS,  # read + write every density once
-        "gradient": 16 * VS,  # read densities, write 3 components
-        "dirichlet_interior": n_int * (8 + 9 * S),  # voxel index, values, conditions, density write
-        "dirichlet_faces": face_elems * 4,
+        "sweep_x": 2 * e * VS, "sweep_y": 2 * e * VS, "sweep_z": 2 * e * VS,  # read + write every density once
+        "gradient": (e + 12) * VS,  # read densities, write 3 FP32 components
+        "dirichlet_interior": n_int * (8 + (5 + e) * S),  # voxel index, values, conditions, density write
+        "dirichlet_faces": face_elems * e,
         # positions + volume, then per substrate: reduced sums (12 B), internalized read/write (8 B),
-        # density read/write (8 B)
-        "secretion": A * (16 + 28 * S),
+        # density read/write
+        "secretion": A * (16 + (20 + 2 * e) * S),
+        "copy": 2 * e * VS,  # calibration: the densities buffer copied once
     }
-    # one step of the fused design: 3 sweeps (24 B) + gradient written from the last sweep (12 B) per
+    # one step of the fused design: 3 sweeps + gradient written from the last sweep (12 B) per
     # voxel*substrate, Dirichlet interior after each of the 3 sweeps and before the first, agents
-    step = 36 * VS + 4 * per["dirichlet_interior"] + per["dirichlet_faces"] + per["secretion"]
+    step = (6 * e + 12) * VS + 4 * per["dirichlet_interior"] + per["dirichlet_faces"] + per["secretion"]
     return per, step
 
 
@@ -99,6 +101,80 @@ def cpu_baseline(cfg, budget_s, max_steps):
     }
 
 
+def profiled_traffic(kernel, storage):
+    """HBM bytes per launch of `kernel` on the default 400^3 x 8 workload, from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated by the copy kernel); None when there is no such summary."""
+    path = os.path.join(ROOT, "profiles", f"traffic_timestep_400cube_8sub_{storage}.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return int(json.loads(block)[kernel]["traffic_bytes"]), os.path.relpath(path, ROOT)
+    except (OSError, IndexError, KeyError, ValueError):
+        return None, None
+
+
+def bench_batch(args):
+    """64 independent microenvironments of uneven shapes and substrate counts advanced as one batch (one launch
+    per sweep axis and one gradient launch over all instances, the step replayed as one HIP graph)."""
+    import parabiofvm_amd
+    from parabiofvm_amd.configs import apply_config, make_batch_config
+
+    t = time.perf_counter()
+    cfgs = make_batch_config(count=args.batch_count)
+    VS = sum(c["updates_per_step"] for c in cfgs)
+    A = sum(int(c["agents"]["volumes"].shape[0]) for c in cfgs)
+    log(f"[bench] batch of {len(cfgs)} generated in {time.perf_counter() - t:.1f}s: {VS / 1e6:.1f}M voxel*substrates, "
+        f"{A} agents")
+    solvers = []
+    for c in cfgs:
+        g = parabiofvm_amd.DeviceSolver(c["dims"], c["mins"], c["maxs"], c["voxel_shape"], c["S"], c["dt"], c["D"],
+                                        c["decay"], np.zeros(c["S"], dtype=np.float32), device=0)
+        apply_config(c, g)
+        g.set_storage(args.storage)
+        g.initialize()
+        g.store()
+        solvers.append(g)
+    batch = parabiofvm_amd.Batch(solvers)
+    for i in range(args.warmup):
+        batch.timestep(i % 10 == 0, True)
+    batch.wait()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        batch.timestep(i % 10 == 0, True)
+    batch.wait()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = VS * args.steps / elapsed
+    log(f"[bench] {args.steps} batch steps in {elapsed:.3f}s -> {ms_per_step:.3f} ms/step, {value:.3e} updates/s")
+    e = 4 if args.storage == "f32" else 2
+    kernel_ms = {}
+    for k in ("sweep_x", "sweep_y", "sweep_z", "gradient", "secretion", "secretion_recompute"):
+        kernel_ms[k] = batch.benchmark(k, args.kernel_iters)
+        log(f"[bench] batched {k}: {kernel_ms[k]:.4f} ms/launch")
+    per = {"sweep_x": 2 * e * VS, "sweep_y": 2 * e * VS, "sweep_z": 2 * e * VS, "gradient": (e + 12) * VS}
+    dominant = max(per, key=lambda k: kernel_ms[k])
+    achieved = per[dominant] / (kernel_ms[dominant] * 1e-3) / 1e9
+    result = {
+        "metric": METRIC, "value": value, "unit": "voxel·substrate updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": (f"batch of {len(cfgs)} microenvironments, edges uniform in [32, 200] per axis, 1..16 "
+                                f"substrates, agents 1% of voxels, {args.storage.upper()} storage, gradient every step, "
+                                f"agent recompute every 10 steps"),
+                   "instances": len(cfgs), "voxel_substrates": VS, "agents": A, "storage": args.storage},
+        "hbm_frac_step": round((6 * e + 12) * VS / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
+        "roofline": {"bound": "hbm", "kernel": "batched " + dominant, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": per[dominant], "avg_launch_ms": round(kernel_ms[dominant], 4)},
+        "cpu_baseline": None,
+    }
+    batch.close()
+    for g in solvers:
+        g.close()
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +188,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--shape", type=int, nargs=3, default=None, help="override the mesh (smaller runs)")
     ap.add_argument("--agents", type=int, default=None)
+    ap.add_argument("--storage", choices=["f32", "bf16", "f16"], default="f32",
+                    help="density storage in HBM between kernels (arithmetic is FP32 in every form)")
+    ap.add_argument("--batch", action="store_true", help="the batched workload: many uneven microenvironments")
+    ap.add_argument("--batch-count", type=int, default=64)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -119,6 +199,9 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path (each GPU box has one MI355X); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.batch:
+        bench_batch(args)
+        return
 
     import parabiofvm_amd
     from parabiofvm_amd.configs import apply_config, make_config
@@ -133,6 +216,7 @@ def main():
     gpu = parabiofvm_amd.DeviceSolver(cfg["dims"], cfg["mins"], cfg["maxs"], cfg["voxel_shape"], S, cfg["dt"],
                                       cfg["D"], cfg["decay"], np.zeros(S, dtype=np.float32), device=0)
     apply_config(cfg, gpu)
+    gpu.set_storage(args.storage)
     gpu.initialize()
     alg = parabiofvm_amd.BFVM_SWEEP_PARTITIONED if args.algorithm == "partitioned" else parabiofvm_amd.BFVM_SWEEP_EXACT
     gpu.set_sweep_algorithm(alg, args.chunk)
@@ -156,33 +240,36 @@ def main():
     log(f"[bench] {args.steps} steps in {elapsed:.3f}s -> {ms_per_step:.3f} ms/step, {value:.3e} updates/s")
 
     # per-kernel averages (HIP events around back-to-back launches on the solver's stream)
-    per_bytes, step_bytes = algorithmic_bytes(cfg)
+    per_bytes, step_bytes = algorithmic_bytes(cfg, 4 if args.storage == "f32" else 2)
     kernel_ms = {}
     for k in ("sweep_x", "sweep_y", "sweep_z", "gradient", "dirichlet_interior", "dirichlet_faces", "secretion",
-              "secretion_recompute"):
+              "secretion_recompute", "copy"):
         kernel_ms[k] = gpu.benchmark(k, args.kernel_iters)
         log(f"[bench] {k}: {kernel_ms[k]:.4f} ms/launch")
     launches_per_step = {"sweep_x": 1, "sweep_y": 1, "sweep_z": 1, "gradient": 1, "dirichlet_interior": 4,
                          "dirichlet_faces": 1, "secretion": 0.9, "secretion_recompute": 0.1}
-    share = {k: kernel_ms[k] * launches_per_step[k] for k in kernel_ms}
+    share = {k: kernel_ms[k] * launches_per_step[k] for k in launches_per_step}
     dominant = max(("sweep_x", "sweep_y", "sweep_z", "gradient"), key=lambda k: share[k])
     achieved = per_bytes[dominant] / (kernel_ms[dominant] * 1e-3) / 1e9
+    traffic, source = (None, None) if (args.shape or args.agents) else profiled_traffic(dominant, args.storage)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": source,
                 "algorithmic_bytes_per_launch": per_bytes[dominant], "avg_launch_ms": round(kernel_ms[dominant], 4)}
 
     result = {
         "metric": METRIC, "value": value, "unit": "voxel·substrate updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": (f"single instance {X}x{Y}x{Z} voxels x {S} substrates, FP32 storage, "
+        "config": {"workload": (f"single instance {X}x{Y}x{Z} voxels x {S} substrates, {args.storage.upper()} storage, "
                                 f"{cfg['agents']['volumes'].shape[0]} agents in 16 clusters, 6 Dirichlet faces + 1% "
                                 f"interior Dirichlet voxels, gradient every step, agent recompute every 10 steps"),
                    "grid": [X, Y, Z], "substrates": S, "agents": int(cfg["agents"]["volumes"].shape[0]),
-                   "sweep": args.algorithm, "partitions": [list(gpu.partition(a)) for a in range(3)]},
+                   "sweep": args.algorithm, "storage": args.storage, "partitions": [list(gpu.partition(a)) for a in range(3)]},
         "hbm_frac_step": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "algorithmic_bytes_per_step": step_bytes,
         "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
+        "copy_ceiling_gbs": round(per_bytes["copy"] / (kernel_ms["copy"] * 1e-3) / 1e9, 1),
         "roofline": roofline,
     }
     gpu.close()

@@ -222,6 +222,28 @@ int bfvm_voxel_positions(bfvm_ctx* ctx, int32_t count, const float* positions, i
 #define BFVM_PHASE_TIMESTEP 14 /* graph replay, recompute = false, with gradient */
 int bfvm_benchmark(bfvm_ctx* ctx, int32_t what, int32_t iterations, float* avg_ms);
 
+/*
+ * Batches of independent microenvironments.  No counterpart in the reference, which runs one
+ * microenvironment per solver (src/solver.h); a batch advances many initialized contexts of any mesh shape
+ * and substrate count with ONE launch per sweep axis and one for the gradient, each driven by a device work
+ * list of (instance, line-group / tile range) items, and replays the whole step (secretion of every
+ * instance -> Dirichlet -> x/y/z sweeps -> gradient) as one captured HIP graph.  Per-instance results equal
+ * what bfvm_timestep gives for that context (same kernels, one substrate per thread, 16-element blocks).
+ *
+ * All contexts must be initialized on one device with one storage form and the partitioned sweep; they
+ * keep their data (store/load through the per-context calls) and must not be re-initialized or destroyed
+ * while the batch exists.  The batch has its own stream: call bfvm_batch_wait before loading results from
+ * a context, and bfvm_wait_for_all(ctx) after an asynchronous context call before the next batch step.
+ */
+typedef struct bfvm_batch bfvm_batch;
+int bfvm_batch_create(bfvm_ctx* const* contexts, int32_t count, bfvm_batch** out);
+int bfvm_batch_destroy(bfvm_batch* batch);
+int bfvm_batch_timestep(bfvm_batch* batch, int32_t recompute, int32_t with_gradient);
+int bfvm_batch_wait(bfvm_batch* batch);
+/* Average ms per launch over all instances: BFVM_KERNEL_SWEEP_X/Y/Z and BFVM_KERNEL_GRADIENT (the batched
+ * launch alone), BFVM_PHASE_SECRETION(_RECOMPUTE), _DIFFUSION, _GRADIENT and _TIMESTEP. */
+int bfvm_batch_benchmark(bfvm_batch* batch, int32_t what, int32_t iterations, float* avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

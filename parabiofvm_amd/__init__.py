@@ -6,4 +6,4 @@ facade of the reference's solver classes lives in parabiofvm_amd/cpp.  This Pyth
 test and benchmark harness over the same C ABI.
 """
 from ._native import BFVM_SWEEP_EXACT, BFVM_SWEEP_PARTITIONED, BfvmError, load  # noqa: F401
-from .solver import DeviceSolver, grid_shape  # noqa: F401
+from .solver import Batch, DeviceSolver, grid_shape  # noqa: F401

@@ -108,6 +108,12 @@ PROTOTYPES = {
                                                ctypes.c_void_p]),
     "bfvm_voxel_positions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
     "bfvm_benchmark": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _F]),
+    "bfvm_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "bfvm_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "bfvm_batch_timestep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "bfvm_batch_wait": (ctypes.c_int, [ctypes.c_void_p]),
+    "bfvm_batch_benchmark": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _F]),
 }
 
 KERNELS = {"sweep_x": 0, "sweep_y": 1, "sweep_z": 2, "gradient": 3, "dirichlet_interior": 4, "dirichlet_faces": 5,

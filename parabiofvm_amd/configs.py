@@ -107,6 +107,21 @@ def make_config(index, seed=1234, shape=None, substrates=None, agents=None, step
     }
 
 
+def make_batch_config(seed=4321, count=64, edge=(32, 200), substrates=(1, 16), agent_fraction=0.01):
+    """Independent microenvironments of uneven shapes: each mesh edge ~ uniform integer in `edge` (independent
+    per axis), substrate count ~ uniform integer in `substrates`, D/decay/initial densities and uniformly placed
+    agents (agent_fraction of the voxel count, at least one) drawn as in the 100^3 workload (index 0), no
+    Dirichlet conditions.  Returns a list of make_config dicts."""
+    rng = _rng(seed)
+    out = []
+    for i in range(count):
+        shape = tuple(int(v) for v in rng.integers(edge[0], edge[1] + 1, size=3))
+        S = int(rng.integers(substrates[0], substrates[1] + 1))
+        A = max(1, int(round(agent_fraction * shape[0] * shape[1] * shape[2])))
+        out.append(make_config(0, seed=seed * 1000 + i, shape=shape, substrates=S, agents=A))
+    return out
+
+
 def apply_config(cfg, backend):
     """Loads a config into a backend with the DeviceSolver / oracle interface (host arrays)."""
     backend.densities[:] = cfg["init"]

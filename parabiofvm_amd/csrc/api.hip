@@ -161,6 +161,41 @@ static void check_gradient(const bfvm_ctx* ctx)
 			throw std::invalid_argument("gradient needs at least 2 voxels along every axis");
 }
 
+// Average ms of `iterations` calls of `one` (enqueueing on `stream`), timed with HIP events on that stream
+// after one untimed call (graph capture, code object loading).
+template <typename F>
+static float time_launches(hipStream_t stream, int iterations, F&& one)
+{
+	one();
+	hipEvent_t start = nullptr, stop = nullptr;
+	BFVM_HIP(hipEventCreate(&start));
+	hipError_t err = hipEventCreate(&stop);
+	if (err != hipSuccess)
+	{
+		(void)hipEventDestroy(start);
+		BFVM_HIP(err);
+	}
+	float ms = 0;
+	try
+	{
+		BFVM_HIP(hipEventRecord(start, stream));
+		for (int i = 0; i < iterations; i++)
+			one();
+		BFVM_HIP(hipEventRecord(stop, stream));
+		BFVM_HIP(hipEventSynchronize(stop));
+		BFVM_HIP(hipEventElapsedTime(&ms, start, stop));
+	}
+	catch (...)
+	{
+		(void)hipEventDestroy(start);
+		(void)hipEventDestroy(stop);
+		throw;
+	}
+	BFVM_HIP(hipEventDestroy(start));
+	BFVM_HIP(hipEventDestroy(stop));
+	return ms / iterations;
+}
+
 static void ensure_partitions(bfvm_ctx* ctx)
 {
 	for (int axis = 0; axis < ctx->mesh.dims; axis++)
@@ -236,6 +271,7 @@ int bfvm_initialize(bfvm_ctx* ctx, const bfvm_microenvironment_desc* desc)
 		free_graphs(ctx);
 		free_agents(ctx);
 		free_fields(ctx);
+		ctx->generation++;
 
 		mesh_info m;
 		m.dims = dims;
@@ -963,35 +999,290 @@ int bfvm_benchmark(bfvm_ctx* ctx, int32_t what, int32_t iterations, float* avg_m
 				throw std::invalid_argument("unknown kernel or phase id");
 			}
 		};
-		// one untimed launch so a graph is captured and code objects are loaded
-		one();
-		hipEvent_t start = nullptr, stop = nullptr;
-		BFVM_HIP(hipEventCreate(&start));
-		hipError_t err = hipEventCreate(&stop);
-		if (err != hipSuccess)
+		*avg_ms = time_launches(ctx->stream, iterations, one);
+		return BFVM_OK;
+	});
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// batches of microenvironments
+// ------------------------------------------------------------------------------------------------
+
+namespace bfvm {
+
+// The per-instance kernels of a batch step run on the batch's stream with the batch's rounding counter.
+struct batch_override
+{
+	bfvm_ctx* c;
+	hipStream_t stream;
+	unsigned* rounding;
+	batch_override(bfvm_ctx* ctx, const bfvm_batch* b) : c(ctx), stream(ctx->stream), rounding(ctx->rounding_step)
+	{
+		c->stream = b->stream;
+		c->rounding_step = b->rounding;
+	}
+	~batch_override()
+	{
+		c->stream = stream;
+		c->rounding_step = rounding;
+	}
+};
+
+static void free_batch(bfvm_batch* b)
+{
+	for (auto& g : b->graphs)
+	{
+		if (g.exec)
+			(void)hipGraphExecDestroy(g.exec);
+		if (g.graph)
+			(void)hipGraphDestroy(g.graph);
+		g = graph_slot {};
+	}
+	for (int a = 0; a < 3; a++)
+		free_batch_sweep(b, a);
+	free_batch_gradient(b);
+	if (b->rounding)
+		(void)hipFree(b->rounding);
+	b->rounding = nullptr;
+	if (b->stream)
+		(void)hipStreamDestroy(b->stream);
+	b->stream = nullptr;
+}
+
+static void check_batch(const bfvm_batch* b)
+{
+	if (!b)
+		throw std::invalid_argument("null batch");
+	for (size_t i = 0; i < b->ctx.size(); i++)
+		if (!b->ctx[i]->initialized || b->ctx[i]->generation != b->generation[i])
+			throw std::logic_error("instance " + std::to_string(i) +
+								   " was re-initialized after bfvm_batch_create; create the batch again");
+}
+
+static uint64_t batch_epoch(const bfvm_batch* b)
+{
+	uint64_t e = b->epoch;
+	for (const bfvm_ctx* c : b->ctx)
+		e += c->epoch;
+	return e;
+}
+
+static void enqueue_batch_sweeps(bfvm_batch* b)
+{
+	if (b->storage != BFVM_STORAGE_F32)
+	{
+		batch_override o(b->ctx[0], b);
+		launch_advance_rounding(b->ctx[0]);
+	}
+	for (bfvm_ctx* c : b->ctx)
+	{
+		batch_override o(c, b);
+		enqueue_dirichlet(c);
+	}
+	for (int axis = 0; axis < 3; axis++)
+	{
+		for (int i : b->axes[axis].single)
 		{
-			(void)hipEventDestroy(start);
-			BFVM_HIP(err);
+			batch_override o(b->ctx[i], b);
+			launch_sweep(b->ctx[i], axis);
 		}
+		launch_batched_sweep(b, axis);
+		for (bfvm_ctx* c : b->ctx)
+			if (axis < c->mesh.dims)
+			{
+				batch_override o(c, b);
+				launch_dirichlet_interior(c);
+			}
+	}
+}
+
+static void enqueue_batch_gradient(bfvm_batch* b)
+{
+	launch_batched_gradient(b);
+	for (int i : b->grad.single)
+	{
+		batch_override o(b->ctx[i], b);
+		launch_gradient(b->ctx[i]);
+	}
+}
+
+static void enqueue_batch_step(bfvm_batch* b, bool recompute, bool with_gradient)
+{
+	for (bfvm_ctx* c : b->ctx)
+	{
+		batch_override o(c, b);
+		launch_cell(c, recompute);
+	}
+	enqueue_batch_sweeps(b);
+	if (with_gradient)
+		enqueue_batch_gradient(b);
+}
+
+} // namespace bfvm
+
+extern "C" {
+
+int bfvm_batch_create(bfvm_ctx* const* contexts, int32_t count, bfvm_batch** out)
+{
+	return guarded("bfvm_batch_create", [&] {
+		if (!contexts || count < 1 || !out)
+			throw std::invalid_argument("need a positive count of contexts and an output pointer");
+		*out = nullptr;
+		auto* b = new bfvm_batch;
 		try
 		{
-			BFVM_HIP(hipEventRecord(start, ctx->stream));
-			for (int i = 0; i < iterations; i++)
-				one();
-			BFVM_HIP(hipEventRecord(stop, ctx->stream));
-			BFVM_HIP(hipEventSynchronize(stop));
-			float ms = 0;
-			BFVM_HIP(hipEventElapsedTime(&ms, start, stop));
-			*avg_ms = ms / iterations;
+			for (int32_t i = 0; i < count; i++)
+			{
+				bfvm_ctx* c = contexts[i];
+				if (!c || !c->initialized)
+					throw std::invalid_argument("context " + std::to_string(i) + " is null or not initialized");
+				if (c->device != contexts[0]->device || c->storage != contexts[0]->storage)
+					throw std::invalid_argument("all contexts of a batch need the same device and storage form");
+				if (c->algorithm != BFVM_SWEEP_PARTITIONED)
+					throw std::invalid_argument("batched sweeps use the partitioned algorithm");
+				for (int32_t j = 0; j < i; j++)
+					if (contexts[j] == c)
+						throw std::invalid_argument("a context appears twice in the batch");
+				b->ctx.push_back(c);
+				b->generation.push_back(c->generation);
+			}
+			b->device = contexts[0]->device;
+			b->storage = contexts[0]->storage;
+			BFVM_HIP(hipSetDevice(b->device));
+			for (bfvm_ctx* c : b->ctx)
+				BFVM_HIP(hipStreamSynchronize(c->stream));
+			BFVM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+			BFVM_HIP(hipMalloc(&b->rounding, sizeof(unsigned)));
+			BFVM_HIP(hipMemsetAsync(b->rounding, 0, sizeof(unsigned), b->stream));
+			for (int a = 0; a < 3; a++)
+				prepare_batch_sweep(b, a);
+			prepare_batch_gradient(b);
+			BFVM_HIP(hipStreamSynchronize(b->stream));
 		}
 		catch (...)
 		{
-			(void)hipEventDestroy(start);
-			(void)hipEventDestroy(stop);
+			free_batch(b);
+			delete b;
 			throw;
 		}
-		BFVM_HIP(hipEventDestroy(start));
-		BFVM_HIP(hipEventDestroy(stop));
+		*out = b;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_batch_destroy(bfvm_batch* batch)
+{
+	return guarded("bfvm_batch_destroy", [&] {
+		if (!batch)
+			return BFVM_OK;
+		BFVM_HIP(hipSetDevice(batch->device));
+		if (batch->stream)
+			BFVM_HIP(hipStreamSynchronize(batch->stream));
+		free_batch(batch);
+		delete batch;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_batch_timestep(bfvm_batch* batch, int32_t recompute, int32_t with_gradient)
+{
+	return guarded("bfvm_batch_timestep", [&] {
+		check_batch(batch);
+		if (with_gradient)
+			for (const bfvm_ctx* c : batch->ctx)
+				check_gradient(c);
+		BFVM_HIP(hipSetDevice(batch->device));
+		auto& slot = batch->graphs[(recompute ? 2 : 0) + (with_gradient ? 1 : 0)];
+		const uint64_t epoch = batch_epoch(batch);
+		if (!slot.exec || slot.epoch != epoch)
+		{
+			if (slot.exec)
+				BFVM_HIP(hipGraphExecDestroy(slot.exec));
+			if (slot.graph)
+				BFVM_HIP(hipGraphDestroy(slot.graph));
+			slot = graph_slot {};
+			BFVM_HIP(hipStreamBeginCapture(batch->stream, hipStreamCaptureModeThreadLocal));
+			try
+			{
+				enqueue_batch_step(batch, recompute != 0, with_gradient != 0);
+			}
+			catch (...)
+			{
+				hipGraph_t g = nullptr;
+				(void)hipStreamEndCapture(batch->stream, &g);
+				if (g)
+					(void)hipGraphDestroy(g);
+				throw;
+			}
+			BFVM_HIP(hipStreamEndCapture(batch->stream, &slot.graph));
+			BFVM_HIP(hipGraphInstantiate(&slot.exec, slot.graph, nullptr, nullptr, 0));
+			slot.epoch = epoch;
+		}
+		BFVM_HIP(hipGraphLaunch(slot.exec, batch->stream));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_batch_wait(bfvm_batch* batch)
+{
+	return guarded("bfvm_batch_wait", [&] {
+		if (!batch)
+			throw std::invalid_argument("null batch");
+		BFVM_HIP(hipSetDevice(batch->device));
+		BFVM_HIP(hipStreamSynchronize(batch->stream));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_batch_benchmark(bfvm_batch* batch, int32_t what, int32_t iterations, float* avg_ms)
+{
+	return guarded("bfvm_batch_benchmark", [&] {
+		check_batch(batch);
+		if (iterations < 1 || !avg_ms)
+			throw std::invalid_argument("iterations must be positive and avg_ms non-null");
+		if (what == BFVM_KERNEL_GRADIENT || what == BFVM_PHASE_TIMESTEP)
+			for (const bfvm_ctx* c : batch->ctx)
+				check_gradient(c);
+		BFVM_HIP(hipSetDevice(batch->device));
+		auto one = [&]() {
+			switch (what)
+			{
+			case BFVM_KERNEL_SWEEP_X:
+			case BFVM_KERNEL_SWEEP_Y:
+			case BFVM_KERNEL_SWEEP_Z:
+				launch_batched_sweep(batch, what - BFVM_KERNEL_SWEEP_X);
+				break;
+			case BFVM_KERNEL_GRADIENT:
+				launch_batched_gradient(batch);
+				break;
+			case BFVM_PHASE_SECRETION:
+			case BFVM_PHASE_SECRETION_RECOMPUTE:
+				for (bfvm_ctx* c : batch->ctx)
+				{
+					batch_override o(c, batch);
+					launch_cell(c, what == BFVM_PHASE_SECRETION_RECOMPUTE);
+				}
+				break;
+			case BFVM_PHASE_DIFFUSION:
+				enqueue_batch_sweeps(batch);
+				break;
+			case BFVM_PHASE_GRADIENT:
+				enqueue_batch_gradient(batch);
+				break;
+			case BFVM_PHASE_TIMESTEP:
+			{
+				int rc = bfvm_batch_timestep(batch, 0, 1);
+				if (rc != BFVM_OK)
+					throw std::runtime_error(g_error);
+				break;
+			}
+			default:
+				throw std::invalid_argument("unknown kernel or phase id for a batch");
+			}
+		};
+		*avg_ms = time_launches(batch->stream, iterations, one);
 		return BFVM_OK;
 	});
 }

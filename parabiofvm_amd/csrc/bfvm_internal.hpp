@@ -243,6 +243,19 @@ struct mesh_info
 	float dt = 0;
 };
 
+// Work-list item of the batched (many microenvironment) kernels: a range of workgroup-sized units
+// (sweep line groups or gradient tiles) of one instance.
+struct batch_item
+{
+	int32_t instance;
+	int32_t pad;
+	int64_t begin, end;
+};
+
+constexpr int batch_chunk = 16;			 // register block of the batched sweeps (one substrate per thread)
+constexpr int batch_group_threads = 256; // workgroup size the batched sweeps aim for
+constexpr int batch_max_threads = 1024;
+
 struct graph_slot
 {
 	hipGraphExec_t exec = nullptr;
@@ -298,7 +311,42 @@ struct bfvm_ctx
 	int* ballots = nullptr;
 	int* conflict = nullptr; // device flag
 
-	uint64_t epoch = 1; // bumps whenever captured kernel arguments would change
+	uint64_t epoch = 1;		 // bumps whenever captured kernel arguments would change
+	uint64_t generation = 0; // bumps at every bfvm_initialize (buffers reallocated)
+	bfvm::graph_slot graphs[4];
+};
+
+// Many microenvironments advanced together (bfvm_batch_*): one launch per sweep axis and one for the
+// gradient over all instances, driven by device work lists; per-instance kernels for the rest.
+struct bfvm_batch
+{
+	int device = 0;
+	hipStream_t stream = nullptr;
+	std::vector<bfvm_ctx*> ctx;
+	std::vector<uint64_t> generation; // each instance's initialization when the batch was made
+	int storage = BFVM_STORAGE_F32;
+	unsigned* rounding = nullptr; // step counter of the stochastic rounding, shared by all instances
+	struct sweep_plan
+	{
+		void* inst = nullptr; // device part_args per instance
+		bfvm::batch_item* items = nullptr;
+		int nitems = 0;
+		size_t lds = 0;
+		int threads = 0;
+		int blocks = 0;
+		std::vector<float*> tables; // coefficient tables owned by the batch
+		std::vector<int> single;	// instances whose mesh has length 1 on this axis (per-instance launch)
+		bool any = false;
+	} axes[3];
+	struct gradient_plan
+	{
+		void* inst = nullptr; // device gradient_args per instance
+		bfvm::batch_item* items = nullptr;
+		int nitems = 0;
+		int blocks = 0;
+		std::vector<int> single; // instances of fewer than 3 dimensions (per-instance launch)
+	} grad;
+	uint64_t epoch = 0;
 	bfvm::graph_slot graphs[4];
 };
 
@@ -327,5 +375,14 @@ void launch_cell_release(bfvm_ctx* ctx, int index);
 void launch_voxel_positions(bfvm_ctx* ctx, int count, const float* d_pos, int* d_out);
 
 int supported_chunk(int request);
+partition choose_partition(const bfvm_ctx* ctx, int axis, int chunk, int lines_request, bool allow_vec);
+
+// batched kernels (bfvm_batch): plans are built once at bfvm_batch_create; launches enqueue on b->stream
+void prepare_batch_sweep(bfvm_batch* b, int axis);
+void launch_batched_sweep(bfvm_batch* b, int axis);
+void free_batch_sweep(bfvm_batch* b, int axis);
+void prepare_batch_gradient(bfvm_batch* b);
+void launch_batched_gradient(bfvm_batch* b);
+void free_batch_gradient(bfvm_batch* b);
 
 } // namespace bfvm

@@ -505,11 +505,13 @@ __device__ __forceinline__ typename vec_of<V>::type faces_slow(typename vec_of<V
 // (a block of 16 float4 values plus coefficients does not fit the 128 of a 1024-thread workgroup).
 constexpr int max_group_threads(int V) { return V == 4 ? 512 : 1024; }
 
+// Groups g_first, g_first + g_step, ... < g_end of one partitioned sweep, by one workgroup.  The caller
+// synchronises the workgroup before reusing `lds`.
 template <int M, int AXIS, int V, int K>
-__global__ __launch_bounds__(max_group_threads(V)) void sweep_partitioned_kernel(part_args a)
+__device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int64_t g_first, int64_t g_end,
+											 int64_t g_step)
 {
 	using vec = typename vec_of<V>::type;
-	extern __shared__ float lds[];
 	const sweep_geometry& g = a.g;
 	const int S = g.S;
 	const int P = a.P;
@@ -542,7 +544,7 @@ __global__ __launch_bounds__(max_group_threads(V)) void sweep_partitioned_kernel
 	}
 	__syncthreads();
 
-	for (int64_t group = blockIdx.x; group < a.groups; group += gridDim.x)
+	for (int64_t group = g_first; group < g_end; group += g_step)
 	{
 		const int64_t line = group * lpg + line_local;
 		const bool active = k >= 0 && line < a.lines;
@@ -687,6 +689,30 @@ __global__ __launch_bounds__(max_group_threads(V)) void sweep_partitioned_kernel
 	}
 }
 
+template <int M, int AXIS, int V, int K>
+__global__ __launch_bounds__(max_group_threads(V)) void sweep_partitioned_kernel(part_args a)
+{
+	extern __shared__ float lds[];
+	sweep_groups<M, AXIS, V, K>(a, lds, blockIdx.x, a.groups, gridDim.x);
+}
+
+// Many microenvironments in one launch: workgroups walk a work list of (instance, group range) items;
+// every instance runs one substrate per thread with 16-element register blocks, and its workgroup shape
+// fits the launch's block size.
+template <int AXIS, int K>
+__global__ __launch_bounds__(batch_max_threads) void sweep_batched_kernel(const part_args* __restrict__ inst,
+																			const batch_item* __restrict__ items,
+																			int nitems)
+{
+	extern __shared__ float lds[];
+	for (int it = blockIdx.x; it < nitems; it += gridDim.x)
+	{
+		const batch_item w = items[it];
+		const part_args a = inst[w.instance];
+		sweep_groups<batch_chunk, AXIS, 1, K>(a, lds, w.begin, w.end, 1);
+	}
+}
+
 // ------------------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------------------
@@ -766,18 +792,16 @@ static size_t partition_lds_bytes(const partition& p, int S, int lpg)
 		   ((size_t)4 * p.ntypes * p.mb * S + (size_t)6 * p.P * S + (size_t)2 * p.P * lpg * p.vec);
 }
 
-void build_partition(bfvm_ctx* ctx, int axis, int chunk, int lines_request)
+partition choose_partition(const bfvm_ctx* ctx, int axis, int chunk, int lines_request, bool allow_vec)
 {
-	auto& t = ctx->axes[axis];
 	const int n = ctx->mesh.grid[axis];
 	const int S = ctx->mesh.S;
-	free_partition_tables(t);
-	t.part = partition {};
-	t.part.n = n;
+	partition none;
+	none.n = n;
 	if (n <= 1)
-		return;
+		return none;
 	// 4 substrates per thread when they pack into 16-byte vectors (register chunk 16: 64 data VGPRs)
-	const bool vec4 = S % 4 == 0;
+	const bool vec4 = allow_vec && S % 4 == 0;
 	// candidate register chunks, smallest first from the request; 16-bit storage instantiates 16 and 32 only
 	const bool wide = ctx->storage != BFVM_STORAGE_F32;
 	std::vector<int> candidates;
@@ -806,38 +830,51 @@ void build_partition(bfvm_ctx* ctx, int axis, int chunk, int lines_request)
 		p.threads = threads;
 		const bool fits = threads <= max_threads && lpg * p.P <= threads &&
 						  partition_lds_bytes(p, S, lpg) <= 64 * 1024 && (p.P == 1 || p.base >= 3);
-		if (fits || last_candidate)
-		{
-			if (!fits)
-				throw std::runtime_error("partitioned sweep: axis too long for the register/LDS budget "
-										 "(use the exact sweep)");
-			t.part = p;
-			break;
-		}
+		if (fits)
+			return p;
+		if (last_candidate)
+			throw std::runtime_error("partitioned sweep: axis too long for the register/LDS budget "
+									 "(use the exact sweep)");
 	}
-	const partition& p = t.part;
-	BFVM_HIP(hipMalloc(&t.coef, sizeof(float) * 4 * p.ntypes * p.mb * S));
-	BFVM_HIP(hipMalloc(&t.red, sizeof(float) * 3 * 2 * p.P * S));
-	BFVM_HIP(hipMalloc(&t.A, sizeof(float) * S));
+}
+
+void alloc_partition_tables(const partition& p, int S, float*& coef, float*& red, float*& A)
+{
+	BFVM_HIP(hipMalloc(&coef, sizeof(float) * 4 * p.ntypes * p.mb * S));
+	BFVM_HIP(hipMalloc(&red, sizeof(float) * 3 * 2 * p.P * S));
+	BFVM_HIP(hipMalloc(&A, sizeof(float) * S));
+}
+
+void build_partition(bfvm_ctx* ctx, int axis, int chunk, int lines_request)
+{
+	auto& t = ctx->axes[axis];
+	free_partition_tables(t);
+	t.part = choose_partition(ctx, axis, chunk, lines_request, true);
+	if (t.part.P > 0)
+		alloc_partition_tables(t.part, ctx->mesh.S, t.coef, t.red, t.A);
+}
+
+void launch_partition_tables(const bfvm_ctx* ctx, int axis, const partition& p, float* coef, float* red, float* A,
+							 hipStream_t stream)
+{
+	if (p.P == 0)
+		return;
+	const int S = ctx->mesh.S;
+	hipLaunchKernelGGL(partition_coefficients_kernel, dim3((S + 63) / 64), dim3(64), 0, stream, coef, red, A,
+					   ctx->d_D, ctx->d_decay, ctx->mesh.dt, ctx->mesh.vs[axis], ctx->mesh.dims, p.n, S, p.P, p.base,
+					   p.rem, p.ntypes, p.mb);
+	BFVM_HIP(hipGetLastError());
 }
 
 void launch_partition_coefficients(bfvm_ctx* ctx, int axis)
 {
 	auto& t = ctx->axes[axis];
-	const partition& p = t.part;
-	if (p.P == 0)
-		return;
-	const int S = ctx->mesh.S;
-	hipLaunchKernelGGL(partition_coefficients_kernel, dim3((S + 63) / 64), dim3(64), 0, ctx->stream, t.coef, t.red,
-					   t.A, ctx->d_D, ctx->d_decay, ctx->mesh.dt, ctx->mesh.vs[axis], ctx->mesh.dims, p.n, S, p.P,
-					   p.base, p.rem, p.ntypes, p.mb);
-	BFVM_HIP(hipGetLastError());
+	launch_partition_tables(ctx, axis, t.part, t.coef, t.red, t.A, ctx->stream);
 }
 
-template <int M, int AXIS, int V, int K>
-static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const axis_tables& t)
+static part_args make_part_args(const sweep_geometry& g, const partition& p, const float* coef, const float* red,
+								const float* A)
 {
-	const partition& p = t.part;
 	part_args a;
 	a.g = g;
 	a.n = p.n;
@@ -846,15 +883,23 @@ static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const a
 	a.rem = p.rem;
 	a.ntypes = p.ntypes;
 	a.mb = p.mb;
-	a.coef = t.coef;
-	a.red = t.red;
-	a.A = t.A;
+	a.coef = coef;
+	a.red = red;
+	a.A = A;
 	const int64_t total = (int64_t)g.X * g.Y * g.Z * g.S;
-	a.lines = total / ((int64_t)p.n * V);
-	const int lpg = p.lpg, threads = p.threads;
-	a.lines_per_group = lpg;
-	a.groups = (a.lines + lpg - 1) / lpg;
-	const size_t lds = partition_lds_bytes(p, g.S, lpg);
+	a.lines = total / ((int64_t)p.n * p.vec);
+	a.lines_per_group = p.lpg;
+	a.groups = (a.lines + p.lpg - 1) / p.lpg;
+	return a;
+}
+
+template <int M, int AXIS, int V, int K>
+static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const axis_tables& t)
+{
+	const partition& p = t.part;
+	const part_args a = make_part_args(g, p, t.coef, t.red, t.A);
+	const int threads = p.threads;
+	const size_t lds = partition_lds_bytes(p, g.S, p.lpg);
 	const int64_t max_blocks = 8192;
 	const int blocks = (int)std::min<int64_t>(a.groups, max_blocks);
 	hipLaunchKernelGGL((sweep_partitioned_kernel<M, AXIS, V, K>), dim3(blocks), dim3(threads), lds, ctx->stream, a);
@@ -961,6 +1006,106 @@ void launch_sweep(bfvm_ctx* ctx, int axis)
 		launch_sweep_storage<store_f32>(ctx, axis);
 		break;
 	}
+}
+
+} // namespace bfvm
+
+namespace bfvm {
+
+// ------------------------------------------------------------------------------------------------
+// batched sweeps
+// ------------------------------------------------------------------------------------------------
+
+void free_batch_sweep(bfvm_batch* b, int axis)
+{
+	auto& ax = b->axes[axis];
+	for (float* t : ax.tables)
+		if (t)
+			BFVM_HIP(hipFree(t));
+	ax.tables.clear();
+	if (ax.inst)
+		BFVM_HIP(hipFree(ax.inst));
+	if (ax.items)
+		BFVM_HIP(hipFree(ax.items));
+	ax = bfvm_batch::sweep_plan {};
+}
+
+// Per instance: the partition with 16-element register blocks, one substrate per thread and as many lines
+// per group as fit batch_group_threads; its own coefficient tables; work items of about 64K elements.
+void prepare_batch_sweep(bfvm_batch* b, int axis)
+{
+	auto& ax = b->axes[axis];
+	std::vector<part_args> args(b->ctx.size());
+	std::vector<batch_item> items;
+	for (size_t i = 0; i < b->ctx.size(); i++)
+	{
+		bfvm_ctx* ctx = b->ctx[i];
+		args[i] = part_args {};
+		if (axis >= ctx->mesh.dims)
+			continue;
+		const int n = ctx->mesh.grid[axis];
+		if (n == 1)
+		{
+			ax.single.push_back((int)i);
+			continue;
+		}
+		const int S = ctx->mesh.S;
+		const int P = (n + batch_chunk - 1) / batch_chunk;
+		const int request = axis == 0 ? std::max(1, batch_group_threads / (S * P))
+									  : std::max(8, batch_group_threads / P / 8 * 8);
+		const partition p = choose_partition(ctx, axis, batch_chunk, request, false);
+		if (p.chunk != batch_chunk || p.vec != 1 || p.threads > batch_max_threads)
+			throw std::runtime_error("batched sweep: instance " + std::to_string(i) +
+									 " does not fit one workgroup of 16-element blocks (axis too long)");
+		float *coef = nullptr, *red = nullptr, *A = nullptr;
+		alloc_partition_tables(p, S, coef, red, A);
+		ax.tables.push_back(coef);
+		ax.tables.push_back(red);
+		ax.tables.push_back(A);
+		launch_partition_tables(ctx, axis, p, coef, red, A, b->stream);
+		sweep_geometry g = geometry(ctx);
+		g.rounding = b->rounding;
+		args[i] = make_part_args(g, p, coef, red, A);
+		ax.lds = std::max(ax.lds, partition_lds_bytes(p, S, p.lpg));
+		ax.threads = std::max(ax.threads, p.threads);
+		const int64_t per_group = (int64_t)p.lpg * n;
+		const int64_t per_item = std::max<int64_t>(1, 65536 / per_group);
+		for (int64_t g0 = 0; g0 < args[i].groups; g0 += per_item)
+			items.push_back(batch_item { (int32_t)i, 0, g0, std::min(args[i].groups, g0 + per_item) });
+		ax.any = true;
+	}
+	if (!ax.any)
+		return;
+	BFVM_HIP(hipMalloc(&ax.inst, sizeof(part_args) * args.size()));
+	BFVM_HIP(hipMemcpy(ax.inst, args.data(), sizeof(part_args) * args.size(), hipMemcpyHostToDevice));
+	BFVM_HIP(hipMalloc(&ax.items, sizeof(batch_item) * items.size()));
+	BFVM_HIP(hipMemcpy(ax.items, items.data(), sizeof(batch_item) * items.size(), hipMemcpyHostToDevice));
+	ax.nitems = (int)items.size();
+	ax.blocks = std::min(ax.nitems, 2048);
+}
+
+template <int AXIS, int K>
+static void launch_batched_t(bfvm_batch* b)
+{
+	const auto& ax = b->axes[AXIS];
+	hipLaunchKernelGGL((sweep_batched_kernel<AXIS, K>), dim3(ax.blocks), dim3(ax.threads), ax.lds, b->stream,
+					   (const part_args*)ax.inst, ax.items, ax.nitems);
+	BFVM_HIP(hipGetLastError());
+}
+
+void launch_batched_sweep(bfvm_batch* b, int axis)
+{
+	if (!b->axes[axis].any)
+		return;
+	by_storage(b->storage, [&](auto kind) {
+		constexpr int K = decltype(kind)::value;
+		if (axis == 0)
+			launch_batched_t<0, K>(b);
+		else if (axis == 1)
+			launch_batched_t<1, K>(b);
+		else
+			launch_batched_t<2, K>(b);
+	});
 }
 
 } // namespace bfvm

@@ -204,40 +204,46 @@ __global__ __launch_bounds__(256) void gradient_kernel(float* __restrict__ grad,
 constexpr int grad_tile = 256;
 constexpr int num_xcds = 8;
 
+// One 3-D gradient field: the densities, the output and the tiling of its (y, q) columns.
+struct gradient_args
+{
+	float* grad;
+	const void* dens;
+	const float* inv;
+	int Y, Z, S, XS, vec;
+	int tiles_per_row, zchunks, zchunk, ntiles;
+	float bx, cx, by, cy, bz, cz;
+	int64_t VS;
+};
+
 template <int V, int K>
-__global__ __launch_bounds__(grad_tile) void gradient3d_kernel(float* __restrict__ grad, const void* __restrict__ dens,
-															   const float* __restrict__ inv, int Y, int Z, int S,
-															   int XS, int tiles_per_row,
-															   int zchunks, int zchunk, int ntiles, int per_xcd,
-															   float bx, float cx, float by, float cy, float bz,
-															   float cz, int64_t VS)
+__device__ __forceinline__ void gradient_tile(const gradient_args& a, int tile)
 {
 	using vec = typename vec_of<V>::type;
-	const int b = blockIdx.x;
-	const int tile = (b % num_xcds) * per_xcd + b / num_xcds;
-	if (tile >= ntiles)
-		return;
 	// z chunk outermost: the tiles one XCD runs together are neighbouring rows of the same z range
-	const int row_tiles = ntiles / zchunks;
+	const int row_tiles = a.ntiles / a.zchunks;
 	const int zc = tile / row_tiles;
 	const int row_tile = tile % row_tiles;
-	const int y = row_tile / tiles_per_row;
-	const int q = ((row_tile % tiles_per_row) * grad_tile + threadIdx.x) * V;
+	const int y = row_tile / a.tiles_per_row;
+	const int q = ((row_tile % a.tiles_per_row) * grad_tile + threadIdx.x) * V;
+	const int S = a.S, XS = a.XS, Y = a.Y, Z = a.Z;
 	if (q >= XS)
 		return;
-	const int z0 = zc * zchunk;
-	const int z1 = min(Z, z0 + zchunk);
+	const int z0 = zc * a.zchunk;
+	const int z1 = min(Z, z0 + a.zchunk);
 	const int64_t plane = (int64_t)Y * XS;
 	const int64_t off = (int64_t)y * XS + q;
-	float* g0 = grad + off;
-	float* g1 = g0 + VS;
-	float* g2 = g1 + VS;
+	float* g0 = a.grad + off;
+	float* g1 = g0 + a.VS;
+	float* g2 = g1 + a.VS;
 	const bool xlo = q < S, xhi = q >= XS - S;
 	const bool ylo = y == 0, yhi = y == Y - 1;
+	const float bx = a.bx, cx = a.cx, by = a.by, cy = a.cy, bz = a.bz, cz = a.cz;
+	const void* dens = a.dens;
 
 	vec is = vec(1.f);
 	if constexpr (K != store_f32)
-		is = vload<V>(inv + q % S);
+		is = vload<V>(a.inv + q % S);
 	auto ld = [&](int64_t o) {
 		const vec d = dload<K, V>(dens, off + o);
 		if constexpr (K == store_f32)
@@ -282,6 +288,35 @@ __global__ __launch_bounds__(grad_tile) void gradient3d_kernel(float* __restrict
 	}
 }
 
+template <int V, int K>
+__global__ __launch_bounds__(grad_tile) void gradient3d_kernel(gradient_args a, int per_xcd)
+{
+	const int b = blockIdx.x;
+	const int tile = (b % num_xcds) * per_xcd + b / num_xcds;
+	if (tile < a.ntiles)
+		gradient_tile<V, K>(a, tile);
+}
+
+// Many microenvironments: workgroups walk (instance, tile range) items; 4 substrates per thread for the
+// instances whose substrate count is a multiple of 4.
+template <int K>
+__global__ __launch_bounds__(grad_tile) void gradient_batched_kernel(const gradient_args* __restrict__ inst,
+																	 const batch_item* __restrict__ items, int nitems)
+{
+	for (int it = blockIdx.x; it < nitems; it += gridDim.x)
+	{
+		const batch_item w = items[it];
+		const gradient_args a = inst[w.instance];
+		for (int64_t t = w.begin; t < w.end; t++)
+		{
+			if (a.vec == 4)
+				gradient_tile<4, K>(a, (int)t);
+			else
+				gradient_tile<1, K>(a, (int)t);
+		}
+	}
+}
+
 // Calibration kernel: float4 copy of n4 16-byte elements.  Its bandwidth is the practical ceiling the
 // solver kernels are compared against, and its counter readings scale the traffic counters.
 __global__ __launch_bounds__(256) void copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src,
@@ -303,6 +338,89 @@ void launch_copy(bfvm_ctx* ctx)
 	BFVM_HIP(hipGetLastError());
 }
 
+// Tiling of one 3-D field: z is split so that there are about `target` tiles (workgroups).
+static gradient_args make_gradient_args(const bfvm_ctx* ctx, int64_t target)
+{
+	const auto& m = ctx->mesh;
+	gradient_args a;
+	a.grad = ctx->grad;
+	a.dens = ctx->dens;
+	a.inv = ctx->dinv;
+	a.Y = m.grid[1];
+	a.Z = m.grid[2];
+	a.S = m.S;
+	a.XS = m.grid[0] * m.S;
+	a.vec = m.S % 4 == 0 ? 4 : 1;
+	a.tiles_per_row = (a.XS / a.vec + grad_tile - 1) / grad_tile;
+	const int64_t row_tiles = (int64_t)a.Y * a.tiles_per_row;
+	int zchunks = (int)std::max<int64_t>(1, std::min<int64_t>(a.Z, (target + row_tiles - 1) / row_tiles));
+	a.zchunk = (a.Z + zchunks - 1) / zchunks;
+	a.zchunks = (a.Z + a.zchunk - 1) / a.zchunk;
+	a.ntiles = (int)(row_tiles * a.zchunks);
+	a.bx = 1.f / m.vs[0];
+	a.cx = 1.f / (m.vs[0] * 2);
+	a.by = 1.f / m.vs[1];
+	a.cy = 1.f / (m.vs[1] * 2);
+	a.bz = 1.f / m.vs[2];
+	a.cz = 1.f / (m.vs[2] * 2);
+	a.VS = (int64_t)m.V * m.S;
+	return a;
+}
+
+void free_batch_gradient(bfvm_batch* b)
+{
+	auto& gp = b->grad;
+	if (gp.inst)
+		BFVM_HIP(hipFree(gp.inst));
+	if (gp.items)
+		BFVM_HIP(hipFree(gp.items));
+	gp = bfvm_batch::gradient_plan {};
+}
+
+// Tiles of 16 z planes; items of 4 tiles (neighbouring rows of one z range, one after the other in one
+// workgroup).
+void prepare_batch_gradient(bfvm_batch* b)
+{
+	auto& gp = b->grad;
+	std::vector<gradient_args> args(b->ctx.size());
+	std::vector<batch_item> items;
+	for (size_t i = 0; i < b->ctx.size(); i++)
+	{
+		const bfvm_ctx* ctx = b->ctx[i];
+		if (ctx->mesh.dims != 3)
+		{
+			gp.single.push_back((int)i);
+			args[i] = gradient_args {};
+			continue;
+		}
+		const int tpr = (ctx->mesh.grid[0] * ctx->mesh.S / (ctx->mesh.S % 4 == 0 ? 4 : 1) + grad_tile - 1) / grad_tile;
+		const int64_t row_tiles = (int64_t)ctx->mesh.grid[1] * tpr;
+		args[i] = make_gradient_args(ctx, row_tiles * ((ctx->mesh.grid[2] + 15) / 16));
+		for (int64_t t0 = 0; t0 < args[i].ntiles; t0 += 4)
+			items.push_back(batch_item { (int32_t)i, 0, t0, std::min<int64_t>(args[i].ntiles, t0 + 4) });
+	}
+	if (items.empty())
+		return;
+	BFVM_HIP(hipMalloc(&gp.inst, sizeof(gradient_args) * args.size()));
+	BFVM_HIP(hipMemcpy(gp.inst, args.data(), sizeof(gradient_args) * args.size(), hipMemcpyHostToDevice));
+	BFVM_HIP(hipMalloc(&gp.items, sizeof(batch_item) * items.size()));
+	BFVM_HIP(hipMemcpy(gp.items, items.data(), sizeof(batch_item) * items.size(), hipMemcpyHostToDevice));
+	gp.nitems = (int)items.size();
+	gp.blocks = std::min(gp.nitems, 8192);
+}
+
+void launch_batched_gradient(bfvm_batch* b)
+{
+	const auto& gp = b->grad;
+	if (gp.nitems == 0)
+		return;
+	by_storage(b->storage, [&](auto kind) {
+		hipLaunchKernelGGL((gradient_batched_kernel<decltype(kind)::value>), dim3(gp.blocks), dim3(grad_tile), 0,
+						   b->stream, (const gradient_args*)gp.inst, gp.items, gp.nitems);
+	});
+	BFVM_HIP(hipGetLastError());
+}
+
 void launch_gradient(bfvm_ctx* ctx)
 {
 	const auto& m = ctx->mesh;
@@ -315,24 +433,13 @@ void launch_gradient(bfvm_ctx* ctx)
 	}
 	if (m.dims == 3)
 	{
-		const int X = m.grid[0], Y = m.grid[1], Z = m.grid[2];
-		const int XS = X * m.S;
-		const int V = m.S % 4 == 0 ? 4 : 1;
-		const int tiles_per_row = (XS / V + grad_tile - 1) / grad_tile;
-		// split z so that there are enough workgroups to fill the chip several times over
-		const int64_t row_tiles = (int64_t)Y * tiles_per_row;
-		const int64_t target = ctx->gradient_groups > 0 ? ctx->gradient_groups : 16384;
-		int zchunks = (int)std::max<int64_t>(1, std::min<int64_t>(Z, (target + row_tiles - 1) / row_tiles));
-		const int zchunk = (Z + zchunks - 1) / zchunks;
-		zchunks = (Z + zchunk - 1) / zchunk;
-		const int64_t ntiles = row_tiles * zchunks;
-		const int per_xcd = (int)((ntiles + num_xcds - 1) / num_xcds);
+		const gradient_args a = make_gradient_args(ctx, ctx->gradient_groups > 0 ? ctx->gradient_groups : 16384);
+		const int per_xcd = (a.ntiles + num_xcds - 1) / num_xcds;
 		by_storage(ctx->storage, [&](auto k) {
 			constexpr int K = decltype(k)::value;
-			auto kernel = V == 4 ? gradient3d_kernel<4, K> : gradient3d_kernel<1, K>;
-			hipLaunchKernelGGL(kernel, dim3((unsigned)(per_xcd * num_xcds)), dim3(grad_tile), 0, ctx->stream,
-							   ctx->grad, ctx->dens, ctx->dinv, Y, Z, m.S, XS, tiles_per_row, zchunks, zchunk,
-							   (int)ntiles, per_xcd, b[0], c[0], b[1], c[1], b[2], c[2], total);
+			auto kernel = a.vec == 4 ? gradient3d_kernel<4, K> : gradient3d_kernel<1, K>;
+			hipLaunchKernelGGL(kernel, dim3((unsigned)(per_xcd * num_xcds)), dim3(grad_tile), 0, ctx->stream, a,
+							   per_xcd);
 		});
 		BFVM_HIP(hipGetLastError());
 		return;

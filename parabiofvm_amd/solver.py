@@ -239,3 +239,41 @@ class DeviceSolver:
         out = np.zeros(count * 3, dtype=np.int32)
         check("bfvm_voxel_positions", self.lib.bfvm_voxel_positions(self.h, count, pos.ctypes.data, out.ctypes.data))
         return out.reshape(count, 3)
+
+
+class Batch:
+    """Many initialized DeviceSolvers advanced together (bfvm_batch_*): one launch per sweep axis and one for
+    the gradient over all instances, the whole step replayed as one captured HIP graph.  Densities are
+    stored and loaded through each solver as before; call wait() before loading."""
+
+    def __init__(self, solvers):
+        self.lib = _native.load()
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        h = ctypes.c_void_p()
+        check("bfvm_batch_create", self.lib.bfvm_batch_create(arr, len(self.solvers), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            check("bfvm_batch_destroy", self.lib.bfvm_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def timestep(self, recompute, with_gradient=True):
+        check("bfvm_batch_timestep",
+              self.lib.bfvm_batch_timestep(self.h, int(bool(recompute)), int(bool(with_gradient))))
+
+    def wait(self):
+        check("bfvm_batch_wait", self.lib.bfvm_batch_wait(self.h))
+
+    def benchmark(self, what, iterations):
+        ms = ctypes.c_float()
+        check("bfvm_batch_benchmark", self.lib.bfvm_batch_benchmark(self.h, _native.KERNELS[what], int(iterations),
+                                                                    ctypes.byref(ms)))
+        return ms.value

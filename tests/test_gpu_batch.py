@@ -1,0 +1,140 @@
+"""Batched microenvironments (bfvm_batch_*) against the oracle run instance by instance (needs an MI355X).
+
+A batch advances every instance with one launch per sweep axis and one gradient launch over all instances
+(work lists of (instance, range) items), replayed as one captured graph per step kind.  Each instance must
+end where the oracle puts it when it steps that instance alone, to the partitioned sweep's bar: relative
+error <= 1e-5 per voxel for densities, gradients relative to the density scale over dx, internalized
+substrates to the cancellation bound of the cell solver tests.
+"""
+import numpy as np
+import pytest
+
+from test_gpu_parity import assert_internalized
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    import parabiofvm_amd
+
+    parabiofvm_amd.load()
+    return parabiofvm_amd
+
+
+def args_of(cfg):
+    return (cfg["dims"], cfg["mins"], cfg["maxs"], cfg["voxel_shape"], cfg["S"], cfg["dt"], cfg["D"], cfg["decay"],
+            np.zeros(cfg["S"], dtype=np.float32))
+
+
+def assert_close(got, exp, what):
+    got = np.asarray(got, dtype=np.float64)
+    exp = np.asarray(exp, dtype=np.float64)
+    err = np.abs(got - exp) / np.maximum(np.abs(exp), 1e-6)
+    i = int(np.argmax(err))
+    assert err.max() <= REL, f"{what}: worst {err.max():.3e} at {i} (gpu {got[i]!r} cpu {exp[i]!r})"
+
+
+def run_batch(pkg, oracle_mod, cfgs, steps, storage="f32"):
+    from parabiofvm_amd.configs import apply_config
+
+    oracles, solvers = [], []
+    for cfg in cfgs:
+        o = oracle_mod.Oracle(*args_of(cfg), dtype=np.float32)
+        g = pkg.DeviceSolver(*args_of(cfg))
+        g.set_storage(storage)
+        for be in (o, g):
+            apply_config(cfg, be)
+            be.initialize()
+        g.store()
+        oracles.append(o)
+        solvers.append(g)
+    batch = pkg.Batch(solvers)
+    for i in range(steps):
+        recompute = i % 10 == 0
+        batch.timestep(recompute, True)
+        for o in oracles:
+            o.timestep(recompute, True)
+    batch.wait()
+    for g in solvers:
+        g.load()
+    return batch, oracles, solvers
+
+
+def check_instances(cfgs, oracles, solvers, steps):
+    for k, (cfg, o, g) in enumerate(zip(cfgs, oracles, solvers)):
+        tag = f"instance {k} grid {cfg['grid']} S {cfg['S']}"
+        assert_close(g.densities, o.densities, f"{tag} densities")
+        scale = max(float(np.abs(o.densities).max()), 1e-6) / min(cfg["voxel_shape"])
+        gerr = np.abs(g.gradients.astype(np.float64) - o.gradients) / (np.abs(o.gradients) + scale)
+        assert gerr.max() <= REL, f"{tag} gradients: worst {gerr.max():.3e}"
+        dmax = max(1.0, float(np.abs(o.densities).max()), float(np.abs(cfg["init"]).max()))
+        assert_internalized(g.internalized, o.internalized, cfg["agents"], cfg["S"], cfg["dt"], steps, dmax,
+                            what=f"{tag} internalized")
+
+
+def test_batch_matches_instances(pkg, oracle_mod):
+    """Eight uneven 3-D instances (substrate counts 1..8, some multiples of 4), 12 steps with two recomputes."""
+    from parabiofvm_amd.configs import make_batch_config
+
+    cfgs = make_batch_config(seed=7, count=8, edge=(6, 40), substrates=(1, 8), agent_fraction=0.02)
+    batch, oracles, solvers = run_batch(pkg, oracle_mod, cfgs, 12)
+    check_instances(cfgs, oracles, solvers, 12)
+    batch.close()
+
+
+def test_batch_mixed_dimensions(pkg, oracle_mod):
+    """A 2-D and a 1-D instance next to 3-D ones: their axes beyond dims and their gradients take the
+    per-instance path inside the same graph."""
+    from parabiofvm_amd.configs import make_batch_config, make_config
+
+    cfgs = make_batch_config(seed=11, count=3, edge=(10, 30), substrates=(2, 5))
+    for dims, shape in ((2, (33, 17, 1)), (1, (45, 1, 1))):
+        c = make_config(0, seed=5 + dims, shape=shape, substrates=3, agents=20)
+        c["dims"] = dims
+        c["maxs"] = [shape[d] * 20 if d < dims else 0 for d in range(3)]
+        c["agents"]["positions"] = c["agents"]["positions"].reshape(-1, 3)[:, :dims].copy().reshape(-1)
+        cfgs.insert(1, c)
+    batch, oracles, solvers = run_batch(pkg, oracle_mod, cfgs, 5)
+    check_instances(cfgs, oracles, solvers, 5)
+    batch.close()
+
+
+def test_batch_refuses_reinitialized_instance(pkg):
+    from parabiofvm_amd.configs import apply_config, make_batch_config
+
+    cfgs = make_batch_config(seed=3, count=2, edge=(8, 12), substrates=(1, 3))
+    solvers = []
+    for cfg in cfgs:
+        g = pkg.DeviceSolver(*args_of(cfg))
+        apply_config(cfg, g)
+        g.initialize()
+        solvers.append(g)
+    batch = pkg.Batch(solvers)
+    batch.timestep(True, True)
+    batch.wait()
+    solvers[1].initialize()
+    with pytest.raises(pkg.BfvmError) as e:
+        batch.timestep(False, True)
+    assert e.value.code == -2
+    batch.close()
+
+
+def test_batch_benchmark_entry_points(pkg):
+    from parabiofvm_amd.configs import apply_config, make_batch_config
+
+    cfgs = make_batch_config(seed=5, count=4, edge=(16, 48), substrates=(1, 16))
+    solvers = []
+    for cfg in cfgs:
+        g = pkg.DeviceSolver(*args_of(cfg))
+        apply_config(cfg, g)
+        g.initialize()
+        g.store()
+        solvers.append(g)
+    batch = pkg.Batch(solvers)
+    for what in ("sweep_x", "sweep_y", "sweep_z", "gradient", "secretion", "secretion_recompute", "diffusion",
+                 "gradient_phase", "timestep"):
+        assert batch.benchmark(what, 2) > 0, what
+    batch.close()

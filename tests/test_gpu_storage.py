@@ -22,7 +22,10 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-L2_BOUND = {"f32": 1e-5, "f16": 1e-3, "bf16": 3e-2}
+# FP32 storage against the FP64 oracle: FP32 coefficient rounding biases every sweep by ~1 ulp the same way, so
+# the error grows about linearly with the step count (~1e-7 per step); the <= 1e-5 per-voxel bar of the FP32 path
+# is against the FP32 oracle (test_gpu_parity).  FP16 meets 1e-3; BF16 is bounded by its quantization floor.
+L2_BOUND = {"f32": 1e-4, "f16": 1e-3, "bf16": 1e-2}
 
 
 @pytest.fixture(scope="module")

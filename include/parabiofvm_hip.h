@@ -74,6 +74,10 @@ typedef struct bfvm_ctx bfvm_ctx;
 #define BFVM_STORAGE_BF16 1
 #define BFVM_STORAGE_F16 2
 
+/* Combining of agents that share a voxel (bfvm_set_secretion_reduction). */
+#define BFVM_REDUCTION_ATOMIC 0 /* per-voxel claim + float atomics into the winner's slot (the reference's scheme) */
+#define BFVM_REDUCTION_SORTED 1 /* agents sorted by voxel at upload, one segmented sum per voxel, no atomics */
+
 /* Sweep algorithm (bfvm_set_sweep_algorithm). */
 #define BFVM_SWEEP_PARTITIONED 0 /* register-resident partitioned (modified) Thomas, FP64-derived coefficients */
 #define BFVM_SWEEP_EXACT 1		 /* one thread per line, the reference's FP32 coefficients and operation order */
@@ -151,6 +155,13 @@ int bfvm_initialize(bfvm_ctx* ctx, const bfvm_microenvironment_desc* desc);
  * run the partitioned sweep only (the exact sweep reports BFVM_ERR_UNSUPPORTED); densities cross the API in
  * FP32 and are converted on the device. */
 int bfvm_set_storage(bfvm_ctx* ctx, int32_t storage);
+
+/* How agents sharing a voxel are combined (BFVM_REDUCTION_*), no counterpart in the reference, whose cell
+ * solvers always claim-and-add (host/cell_solver.cpp:87-135).  The sorted form orders the agents by voxel
+ * when they are uploaded (rocPRIM radix sort) and sums each voxel's run in agent order with the lowest agent
+ * index as the winner, so it is deterministic and free of contended atomics.  Takes effect at the next
+ * bfvm_store_agents; default BFVM_REDUCTION_ATOMIC. */
+int bfvm_set_secretion_reduction(bfvm_ctx* ctx, int32_t mode);
 
 /* Selects the sweep algorithm (default BFVM_SWEEP_PARTITIONED) and, for the partitioned one, the
  * maximum chunk length per thread (0 = automatic).  Takes effect at the next solve. */

@@ -8,13 +8,15 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libparabiofvm_hip.so")
+# PARABIOFVM_HIP_LIB selects another build of the same library (e.g. one compiled with other tuning flags).
+LIB_PATH = os.environ.get("PARABIOFVM_HIP_LIB") or os.path.join(HERE, "libparabiofvm_hip.so")
 
 BFVM_SWEEP_PARTITIONED = 0
 BFVM_SWEEP_EXACT = 1
 BFVM_STORAGE_F32 = 0
 BFVM_STORAGE_BF16 = 1
 BFVM_STORAGE_F16 = 2
+REDUCTION = {"atomic": 0, "sorted": 1}
 STORAGE = {"f32": BFVM_STORAGE_F32, "bf16": BFVM_STORAGE_BF16, "f16": BFVM_STORAGE_F16}
 
 
@@ -86,6 +88,7 @@ PROTOTYPES = {
     "bfvm_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "bfvm_initialize": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MicroenvironmentDesc)]),
     "bfvm_set_storage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "bfvm_set_secretion_reduction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "bfvm_set_sweep_algorithm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "bfvm_store_densities": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "bfvm_store_densities_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

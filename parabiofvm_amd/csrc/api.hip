@@ -82,6 +82,13 @@ static void free_agents(bfvm_ctx* ctx)
 		dfree(*p);
 	dfree(a.vox);
 	dfree(a.win);
+	dfree(a.keys);
+	dfree(a.idx);
+	if (a.sort_tmp)
+		BFVM_HIP(hipFree(a.sort_tmp));
+	a.sort_tmp = nullptr;
+	a.sort_tmp_bytes = 0;
+	a.sorted = false;
 	a.capacity = 0;
 	a.count = 0;
 }
@@ -446,6 +453,18 @@ int bfvm_set_storage(bfvm_ctx* ctx, int32_t storage)
 	});
 }
 
+int bfvm_set_secretion_reduction(bfvm_ctx* ctx, int32_t mode)
+{
+	return guarded("bfvm_set_secretion_reduction", [&] {
+		if (!ctx)
+			throw std::invalid_argument("null context");
+		if (mode != BFVM_REDUCTION_ATOMIC && mode != BFVM_REDUCTION_SORTED)
+			throw std::invalid_argument("unknown reduction mode");
+		ctx->reduction = mode;
+		return BFVM_OK;
+	});
+}
+
 int bfvm_set_sweep_algorithm(bfvm_ctx* ctx, int32_t algorithm, int32_t chunk)
 {
 	return guarded("bfvm_set_sweep_algorithm", [&] {
@@ -659,6 +678,15 @@ int bfvm_store_agents(bfvm_ctx* ctx, const bfvm_agents_desc* agents)
 			dalloc(a.vox, A);
 			dalloc(a.win, A);
 		}
+		const bool sorted = ctx->reduction == BFVM_REDUCTION_SORTED && A > 0;
+		if (sorted && !a.keys)
+		{
+			dalloc(a.keys, 2 * a.capacity);
+			dalloc(a.idx, 2 * a.capacity);
+		}
+		if (sorted != a.sorted)
+			ctx->epoch++;
+		a.sorted = sorted;
 		if ((int32_t)A != a.count)
 			ctx->epoch++;
 		a.count = (int32_t)A;
@@ -678,8 +706,11 @@ int bfvm_store_agents(bfvm_ctx* ctx, const bfvm_agents_desc* agents)
 				BFVM_HIP(hipMemsetAsync(a.frac, 0, sizeof(float) * A * S, ctx->stream));
 			up(a.vol, agents->volumes, A);
 			up(a.pos, agents->positions, A * dims);
-			// positions change only through this upload: the agent -> voxel table is built here
+			// positions change only through this upload: the agent -> voxel table (and the voxel order) is
+			// built here
 			launch_cell_voxels(ctx);
+			if (a.sorted)
+				launch_cell_sort(ctx);
 		}
 		BFVM_HIP(hipStreamSynchronize(ctx->stream));
 		return BFVM_OK;

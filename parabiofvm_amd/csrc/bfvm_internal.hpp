@@ -229,6 +229,12 @@ struct agents_buffers
 	float *rnum = nullptr, *rden = nullptr, *rfac = nullptr;
 	int64_t* vox = nullptr; // voxel of each agent (-1 outside the mesh), computed at upload
 	int* win = nullptr;		// ballot winner whose slot collects each agent's terms
+	// sorted reduction: agents ordered by voxel at upload (keys/idx hold the sort's input and output halves)
+	bool sorted = false;
+	uint64_t* keys = nullptr; // [2][capacity]
+	int* idx = nullptr;		  // [2][capacity]
+	void* sort_tmp = nullptr;
+	size_t sort_tmp_bytes = 0;
 };
 
 struct mesh_info
@@ -308,6 +314,7 @@ struct bfvm_ctx
 	int gradient_groups = 0; // workgroups the 3-D gradient aims for (0 = automatic)
 
 	bfvm::agents_buffers ag;
+	int reduction = BFVM_REDUCTION_ATOMIC; // combining of same-voxel agents, applied at the next agents upload
 	int* ballots = nullptr;
 	int* conflict = nullptr; // device flag
 
@@ -371,6 +378,7 @@ void launch_advance_rounding(bfvm_ctx* ctx);
 
 void launch_cell(bfvm_ctx* ctx, bool recompute);
 void launch_cell_voxels(bfvm_ctx* ctx);
+void launch_cell_sort(bfvm_ctx* ctx); // sorted reduction: order the agents by voxel (upload time)
 void launch_cell_release(bfvm_ctx* ctx, int index);
 void launch_voxel_positions(bfvm_ctx* ctx, int count, const float* d_pos, int* d_out);
 

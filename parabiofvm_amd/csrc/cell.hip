@@ -17,6 +17,8 @@
 //
 // Agents whose position maps outside the grid (or is not finite) get voxel -1 and are ignored; the
 // reference would index out of bounds (src/mesh.cpp:47-53 does not clamp).
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "bfvm_internal.hpp"
 
 namespace bfvm {
@@ -131,6 +133,83 @@ __global__ __launch_bounds__(256) void cell_sum_kernel(int A, int S, const int* 
 	atomicAdd(&rnum[t], num[k]);
 	atomicAdd(&rden[t], w == (int)i ? den[k] + 1 : den[k]);
 	atomicAdd(&rfac[t], fac[k]);
+}
+
+// Sorted reduction.  At upload the agents are radix-sorted by voxel (stable, so a voxel's agents keep
+// ascending index order; agents outside the mesh get the key V and sort last).  The first agent of each
+// voxel's run is its winner.  At recompute the winner's thread sums the run in that order, starting from
+// the winner's own terms with den + 1 as in the claim scheme, and writes the winner's slot directly.
+__global__ __launch_bounds__(256) void cell_keys_kernel(int A, const int64_t* __restrict__ vox, uint64_t V,
+														uint64_t* __restrict__ keys, int* __restrict__ idx)
+{
+	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= A)
+		return;
+	keys[i] = vox[i] < 0 ? V : (uint64_t)vox[i];
+	idx[i] = (int)i;
+}
+
+// win[i] = the first agent of i's run (-1 outside the mesh)
+__global__ __launch_bounds__(256) void cell_runs_kernel(int A, uint64_t V, const uint64_t* __restrict__ keys,
+														const int* __restrict__ idx, int* __restrict__ win)
+{
+	const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= A)
+		return;
+	const uint64_t key = keys[p];
+	if (key == V)
+	{
+		win[idx[p]] = -1;
+		return;
+	}
+	if (p > 0 && keys[p - 1] == key)
+		return;
+	const int w = idx[p];
+	for (int64_t q = p; q < A && keys[q] == key; q++)
+		win[idx[q]] = w;
+}
+
+// recompute with the sorted order: one thread per (sorted position, s); run heads sum their run.
+__global__ __launch_bounds__(256) void cell_sorted_sum_kernel(int A, int S, uint64_t V,
+															  const uint64_t* __restrict__ keys,
+															  const int* __restrict__ idx,
+															  const float* __restrict__ num,
+															  const float* __restrict__ den,
+															  const float* __restrict__ fac, float* __restrict__ rnum,
+															  float* __restrict__ rden, float* __restrict__ rfac,
+															  int* __restrict__ ballots, int* __restrict__ conflict)
+{
+	const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= (int64_t)A * S)
+		return;
+	const int64_t p = t / S;
+	const int s = (int)(t - p * S);
+	const uint64_t key = keys[p];
+	if (key == V || (p > 0 && keys[p - 1] == key))
+		return;
+	const int w = idx[p];
+	const int64_t kw = (int64_t)w * S + s;
+	float sn = 0.f, sd = 0.f, sf = 0.f;
+	sn = sn + num[kw];
+	sd = sd + (den[kw] + 1);
+	sf = sf + fac[kw];
+	int64_t q = p + 1;
+	for (; q < A && keys[q] == key; q++)
+	{
+		const int64_t k = (int64_t)idx[q] * S + s;
+		sn = sn + num[k];
+		sd = sd + den[k];
+		sf = sf + fac[k];
+	}
+	rnum[kw] = sn;
+	rden[kw] = sd;
+	rfac[kw] = sf;
+	if (s == 0)
+	{
+		ballots[key] = w;
+		if (q - p > 1)
+			atomicOr(conflict, 1);
+	}
 }
 
 // Apply, part 1 (host/cell_solver.cpp:137-152, 173-195, 208-234).  One thread per (agent, substrate).
@@ -269,6 +348,40 @@ void launch_cell_voxels(bfvm_ctx* ctx)
 	BFVM_HIP(hipGetLastError());
 }
 
+// Keys and agent indices in the first half of a.keys / a.idx, sorted into the second half; rocPRIM's
+// temporary storage grows on demand.  Radix passes cover only the bits a voxel index (or V) needs.
+void launch_cell_sort(bfvm_ctx* ctx)
+{
+	auto& a = ctx->ag;
+	const int A = a.count;
+	if (A == 0)
+		return;
+	const uint64_t V = ctx->mesh.V;
+	hipLaunchKernelGGL(cell_keys_kernel, dim3((A + 255) / 256), dim3(256), 0, ctx->stream, A, a.vox, V, a.keys, a.idx);
+	BFVM_HIP(hipGetLastError());
+	int bits = 1;
+	while (bits < 64 && (V >> bits) != 0)
+		bits++;
+	size_t need = 0;
+	BFVM_HIP(rocprim::radix_sort_pairs(nullptr, need, a.keys, a.keys + a.capacity, a.idx, a.idx + a.capacity,
+									   (size_t)A, 0, bits, ctx->stream));
+	if (need > a.sort_tmp_bytes)
+	{
+		if (a.sort_tmp)
+			BFVM_HIP(hipFree(a.sort_tmp));
+		a.sort_tmp = nullptr;
+		a.sort_tmp_bytes = 0;
+		BFVM_HIP(hipMalloc(&a.sort_tmp, need));
+		a.sort_tmp_bytes = need;
+	}
+	size_t have = a.sort_tmp_bytes;
+	BFVM_HIP(rocprim::radix_sort_pairs(a.sort_tmp, have, a.keys, a.keys + a.capacity, a.idx, a.idx + a.capacity,
+									   (size_t)A, 0, bits, ctx->stream));
+	hipLaunchKernelGGL(cell_runs_kernel, dim3((A + 255) / 256), dim3(256), 0, ctx->stream, A, V, a.keys + a.capacity,
+					   a.idx + a.capacity, a.win);
+	BFVM_HIP(hipGetLastError());
+}
+
 void launch_cell(bfvm_ctx* ctx, bool recompute)
 {
 	const auto& a = ctx->ag;
@@ -286,12 +399,22 @@ void launch_cell(bfvm_ctx* ctx, bool recompute)
 		BFVM_HIP(hipGetLastError());
 		if (A > 0)
 		{
-			hipLaunchKernelGGL(cell_ballot_kernel, dim3(agent_blocks), dim3(256), 0, ctx->stream, A, a.vox,
-							   ctx->ballots, a.win, ctx->conflict);
-			BFVM_HIP(hipGetLastError());
-			hipLaunchKernelGGL(cell_sum_kernel, dim3(pair_blocks), dim3(256), 0, ctx->stream, A, S, a.win, a.num,
-							   a.den, a.fac, a.rnum, a.rden, a.rfac);
-			BFVM_HIP(hipGetLastError());
+			if (a.sorted)
+			{
+				hipLaunchKernelGGL(cell_sorted_sum_kernel, dim3(pair_blocks), dim3(256), 0, ctx->stream, A, S,
+								   (uint64_t)ctx->mesh.V, a.keys + a.capacity, a.idx + a.capacity, a.num, a.den, a.fac,
+								   a.rnum, a.rden, a.rfac, ctx->ballots, ctx->conflict);
+				BFVM_HIP(hipGetLastError());
+			}
+			else
+			{
+				hipLaunchKernelGGL(cell_ballot_kernel, dim3(agent_blocks), dim3(256), 0, ctx->stream, A, a.vox,
+								   ctx->ballots, a.win, ctx->conflict);
+				BFVM_HIP(hipGetLastError());
+				hipLaunchKernelGGL(cell_sum_kernel, dim3(pair_blocks), dim3(256), 0, ctx->stream, A, S, a.win,
+								   a.num, a.den, a.fac, a.rnum, a.rden, a.rfac);
+				BFVM_HIP(hipGetLastError());
+			}
 		}
 	}
 	if (A == 0)

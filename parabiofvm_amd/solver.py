@@ -110,6 +110,12 @@ class DeviceSolver:
         code = _native.STORAGE[storage] if isinstance(storage, str) else int(storage)
         check("bfvm_set_storage", self.lib.bfvm_set_storage(self.h, code))
 
+    def set_secretion_reduction(self, mode):
+        """How agents sharing a voxel are combined: "atomic" (claim + float atomics) or "sorted" (agents ordered
+        by voxel at upload, segmented sums); applies at the next store/store_agents."""
+        code = _native.REDUCTION[mode] if isinstance(mode, str) else int(mode)
+        check("bfvm_set_secretion_reduction", self.lib.bfvm_set_secretion_reduction(self.h, code))
+
     def set_sweep_algorithm(self, algorithm, chunk=0):
         self._algorithm = (int(algorithm), int(chunk))
         check("bfvm_set_sweep_algorithm", self.lib.bfvm_set_sweep_algorithm(self.h, *self._algorithm))

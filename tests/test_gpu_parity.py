@@ -285,12 +285,16 @@ def agents_for(p, A, seed, crowd=False):
     }
 
 
+@pytest.mark.parametrize("reduction", ["atomic", "sorted"])
 @pytest.mark.parametrize("ci", [True, False])
 @pytest.mark.parametrize("crowd", [False, True])
 @pytest.mark.parametrize("dims,shape", [(3, (20, 15, 10)), (2, (30, 20, 1)), (1, (40, 1, 1))])
-def test_gpu_cell_solver(dims, shape, crowd, ci, pkg, oracle_mod):
+def test_gpu_cell_solver(dims, shape, crowd, ci, reduction, pkg, oracle_mod):
+    """Secretion/uptake with and without internalization, sparse and crowded voxels, both ways of combining
+    agents that share a voxel (claim + atomics, or sorted by voxel with segmented sums)."""
     p = random_problem(dims, shape, 3, seed=53, faces=False, interior=False)
     o, g = build_pair(pkg, oracle_mod, p)
+    g.set_secretion_reduction(reduction)
     a = agents_for(p, 400, seed=59, crowd=crowd)
     oa = o.set_agents(a)
     g.set_agents(a)
@@ -320,8 +324,9 @@ def test_gpu_cell_solver(dims, shape, crowd, ci, pkg, oracle_mod):
                             what="internalized after release")
 
 
-@pytest.mark.parametrize("algorithm", ["partitioned", "exact"])
-def test_gpu_timestep_graph(algorithm, pkg, oracle_mod):
+@pytest.mark.parametrize("algorithm,reduction", [("partitioned", "atomic"), ("exact", "atomic"),
+                                                 ("partitioned", "sorted")])
+def test_gpu_timestep_graph(algorithm, reduction, pkg, oracle_mod):
     """Ten timesteps (secretion -> diffusion -> gradient, recompute every 10 steps as in main.cpp:100) replayed
     from the captured HIP graph, against the oracle stepping the same order."""
     from parabiofvm_amd.configs import apply_config, make_config
@@ -336,6 +341,7 @@ def test_gpu_timestep_graph(algorithm, pkg, oracle_mod):
         be.initialize()
     alg = pkg.BFVM_SWEEP_PARTITIONED if algorithm == "partitioned" else pkg.BFVM_SWEEP_EXACT
     g.set_sweep_algorithm(alg, 0)
+    g.set_secretion_reduction(reduction)
     g.store()
     for i in range(10):
         o.timestep(i % 10 == 0, True)
@@ -364,3 +370,37 @@ def test_gpu_partitioned_tuning(axis, chunk, lines, pkg, oracle_mod):
     g.diffusion()
     g.load()
     assert_close(g.densities, o.densities, what=f"axis {axis} chunk {chunk} lines {lines}: {g.partition(axis)}")
+
+
+def test_gpu_sorted_reduction_agents_outside_and_reupload(pkg, oracle_mod):
+    """Sorted reduction with agents outside the mesh (ignored), then a second upload that moves agents and
+    switches back to the claim scheme: both uploads rebuild the agent -> voxel tables."""
+    p = random_problem(3, (12, 9, 7), 2, seed=71, faces=False, interior=False)
+    o, g = build_pair(pkg, oracle_mod, p)
+    a = agents_for(p, 300, seed=73, crowd=True)
+    pos = a["positions"].reshape(-1, 3)
+    pos[:20] = np.array(p["mins"]) - 50.0  # outside
+    for be in (o, g):
+        be.set_agents(a)
+        be.set_compute_internalized(True)
+    g.set_secretion_reduction("sorted")
+    o.initialize()
+    g.initialize()
+    g.store()
+    for rc in (True, False):
+        o.cell(rc)
+        g.cell(rc)
+    g.load()
+    assert_close(g.densities, o.densities, what="sorted")
+    a2 = agents_for(p, 250, seed=79, crowd=True)
+    for be in (o, g):
+        be.set_agents(a2)
+    g.set_secretion_reduction("atomic")
+    o.initialize()
+    g.store_agents()
+    g.densities[:] = o.densities
+    g.store()
+    o.cell(True)
+    g.cell(True)
+    g.load()
+    assert_close(g.densities, o.densities, what="re-uploaded")

@@ -180,7 +180,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--algorithm", choices=["partitioned", "exact"], default="partitioned")
+    ap.add_argument("--algorithm", choices=["partitioned", "exact", "staged"], default="partitioned")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--kernel-iters", type=int, default=20)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline stepping")
@@ -218,7 +218,8 @@ def main():
     apply_config(cfg, gpu)
     gpu.set_storage(args.storage)
     gpu.initialize()
-    alg = parabiofvm_amd.BFVM_SWEEP_PARTITIONED if args.algorithm == "partitioned" else parabiofvm_amd.BFVM_SWEEP_EXACT
+    alg = {"partitioned": parabiofvm_amd.BFVM_SWEEP_PARTITIONED, "exact": parabiofvm_amd.BFVM_SWEEP_EXACT,
+           "staged": parabiofvm_amd.BFVM_SWEEP_STAGED}[args.algorithm]
     gpu.set_sweep_algorithm(alg, args.chunk)
     gpu.store()
     log(f"[bench] initialized; partitions {[gpu.partition(a) for a in range(3)]}")

@@ -81,6 +81,7 @@ typedef struct bfvm_ctx bfvm_ctx;
 /* Sweep algorithm (bfvm_set_sweep_algorithm). */
 #define BFVM_SWEEP_PARTITIONED 0 /* register-resident partitioned (modified) Thomas, FP64-derived coefficients */
 #define BFVM_SWEEP_EXACT 1		 /* one thread per line, the reference's FP32 coefficients and operation order */
+#define BFVM_SWEEP_STAGED 2		 /* the partitioned algorithm on tiles staged through LDS, one line per thread */
 
 /*
  * Microenvironment description: the fields of `microenvironment` (include/BioFVM/microenvironment.h:19-63)

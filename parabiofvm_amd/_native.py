@@ -13,6 +13,7 @@ LIB_PATH = os.environ.get("PARABIOFVM_HIP_LIB") or os.path.join(HERE, "libparabi
 
 BFVM_SWEEP_PARTITIONED = 0
 BFVM_SWEEP_EXACT = 1
+BFVM_SWEEP_STAGED = 2
 BFVM_STORAGE_F32 = 0
 BFVM_STORAGE_BF16 = 1
 BFVM_STORAGE_F16 = 2

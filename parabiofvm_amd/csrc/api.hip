@@ -470,8 +470,12 @@ int bfvm_set_sweep_algorithm(bfvm_ctx* ctx, int32_t algorithm, int32_t chunk)
 	return guarded("bfvm_set_sweep_algorithm", [&] {
 		if (!ctx)
 			throw std::invalid_argument("null context");
-		if (algorithm != BFVM_SWEEP_PARTITIONED && algorithm != BFVM_SWEEP_EXACT)
+		if (algorithm != BFVM_SWEEP_PARTITIONED && algorithm != BFVM_SWEEP_EXACT && algorithm != BFVM_SWEEP_STAGED)
 			throw std::invalid_argument("unknown sweep algorithm");
+		if (algorithm == BFVM_SWEEP_STAGED && chunk != 0 && chunk != batch_chunk)
+			throw std::invalid_argument("the staged sweep uses 16-element blocks (chunk 0 or 16)");
+		if (algorithm == BFVM_SWEEP_STAGED)
+			chunk = batch_chunk;
 		if (chunk < 0 || chunk > 64)
 			throw std::invalid_argument("chunk must be in [0, 64]");
 		BFVM_HIP(hipSetDevice(ctx->device));
@@ -1171,7 +1175,7 @@ int bfvm_batch_create(bfvm_ctx* const* contexts, int32_t count, bfvm_batch** out
 					throw std::invalid_argument("context " + std::to_string(i) + " is null or not initialized");
 				if (c->device != contexts[0]->device || c->storage != contexts[0]->storage)
 					throw std::invalid_argument("all contexts of a batch need the same device and storage form");
-				if (c->algorithm != BFVM_SWEEP_PARTITIONED)
+				if (c->algorithm == BFVM_SWEEP_EXACT)
 					throw std::invalid_argument("batched sweeps use the partitioned algorithm");
 				for (int32_t j = 0; j < i; j++)
 					if (contexts[j] == c)

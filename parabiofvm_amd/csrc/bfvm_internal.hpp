@@ -132,9 +132,11 @@ __device__ __forceinline__ uint32_t rounding_key(const unsigned* counter, uint32
 // Store of values that accumulate small updates step after step (sweeps, secretion): 16-bit forms round
 // stochastically — a uniform random integer below one unit of the dropped bits is added to the FP32 bit
 // pattern and the dropped bits are cleared, so the rounding is unbiased and an update smaller than half a
-// 16-bit ulp is kept in expectation instead of being lost every step.  The random bits hash the element
-// index with `key`.  BF16 drops 16 bits; FP16 (values pre-scaled into its normal range) drops 13.  The
-// truncated value is exactly representable, so the final conversion is exact.  FP32 stores unchanged.
+// 16-bit ulp is kept in expectation instead of being lost every step.  BF16 drops 16 bits; FP16 (values
+// pre-scaled into its normal range) drops 13.  The truncated value is exactly representable, so the final
+// conversion is exact.  Random bits: one hash of the element index with `key` per vector, split into 16-bit
+// halves, and a multiplicative-congruential step (an odd multiplier, a bijection of the 32-bit state) for
+// every further pair of components.  FP32 stores unchanged.
 template <int K, int V>
 __device__ __forceinline__ void dstore_sr(void* base, int64_t idx, typename vec_of<V>::type x, uint32_t key)
 {
@@ -143,12 +145,14 @@ __device__ __forceinline__ void dstore_sr(void* base, int64_t idx, typename vec_
 	else
 	{
 		constexpr uint32_t drop = K == store_bf16 ? 0xFFFFu : 0x1FFFu;
-		const uint32_t h = (uint32_t)idx ^ ((uint32_t)((uint64_t)idx >> 32) * 0x9E3779B9u) ^ key;
+		uint32_t h = mix32((uint32_t)idx ^ ((uint32_t)((uint64_t)idx >> 32) * 0x9E3779B9u) ^ key);
 		typename vec_of<V>::type y;
 #pragma unroll
 		for (int c = 0; c < V; c++)
 		{
-			const uint32_t r = mix32(h + (uint32_t)c);
+			if (c > 0 && (c & 1) == 0)
+				h = h * 0x2C1B3C6Du + 0x297A2D39u;
+			const uint32_t r = (c & 1) ? (h >> 16) : h;
 			const uint32_t b = (__float_as_uint(comp(x, c)) + (r & drop)) & ~drop;
 			set_comp(y, c, __uint_as_float(b));
 		}
@@ -258,7 +262,7 @@ struct batch_item
 	int64_t begin, end;
 };
 
-constexpr int batch_chunk = 16;			 // register block of the batched sweeps (one substrate per thread)
+constexpr int batch_chunk = 16; // register block of the staged and batched sweeps (one line per thread)
 constexpr int batch_group_threads = 256; // workgroup size the batched sweeps aim for
 constexpr int batch_max_threads = 1024;
 
@@ -383,7 +387,7 @@ void launch_cell_release(bfvm_ctx* ctx, int index);
 void launch_voxel_positions(bfvm_ctx* ctx, int count, const float* d_pos, int* d_out);
 
 int supported_chunk(int request);
-partition choose_partition(const bfvm_ctx* ctx, int axis, int chunk, int lines_request, bool allow_vec);
+partition choose_partition(const bfvm_ctx* ctx, int axis, int chunk, int lines_request, int max_vec);
 
 // batched kernels (bfvm_batch): plans are built once at bfvm_batch_create; launches enqueue on b->stream
 void prepare_batch_sweep(bfvm_batch* b, int axis);

@@ -193,7 +193,7 @@ __global__ void partition_coefficients_kernel(float* __restrict__ coef, float* _
 			inv = 1.0 / (1.0 - L * cprev);
 			cp = U * inv;
 		}
-		red[0 * rstride + (size_t)q * S + s] = (float)L;
+		red[0 * rstride + (size_t)q * S + s] = (float)(L * inv);
 		red[1 * rstride + (size_t)q * S + s] = (float)inv;
 		red[2 * rstride + (size_t)q * S + s] = (float)cp;
 		cprev = cp;
@@ -412,6 +412,7 @@ struct part_args
 	int64_t lines;		 // number of lines (groups of V substrates)
 	int lines_per_group; // lines handled by one workgroup iteration
 	int64_t groups;
+	int vec; // substrates per thread (V)
 };
 
 // Line geometry with V substrates per line.  x: line = (row, substrate group); y: line = (z, q0/V) with
@@ -511,6 +512,8 @@ template <int M, int AXIS, int V, int K>
 __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int64_t g_first, int64_t g_end,
 											 int64_t g_step)
 {
+	// this path is held to a tolerance, not to the reference's operation order: multiply-adds may fuse
+#pragma clang fp contract(fast)
 	using vec = typename vec_of<V>::type;
 	const sweep_geometry& g = a.g;
 	const int S = g.S;
@@ -573,7 +576,8 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 				for (int j = 0; j < M; j++)
 					v[j] = v[j] * inv;
 			}
-			// forward elimination inside the block
+			// forward elimination inside the block: v_j = r_j (v_j - A v_{j-1}), written as r_j v_j - (r_j A) v_{j-1}
+			// so that the recurrence carries one multiply-add
 			const vec As = vload<V>(a.A + s0);
 #pragma unroll
 			for (int j = 0; j < M; j++)
@@ -584,7 +588,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 					if (j < 2)
 						v[j] = r * v[j];
 					else
-						v[j] = r * (v[j] - As * v[j - 1]);
+						v[j] = r * v[j] - (r * As) * v[j - 1];
 				}
 			}
 			// backward elimination towards the block's last unknown
@@ -602,7 +606,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 		}
 		__syncthreads();
 		// reduced system of the line, one thread per line; the LDS reads of step q+1 are issued before the
-		// arithmetic of step q so the recurrence waits on arithmetic latency only
+		// arithmetic of step q, and the table holds L*inv, so each step of the recurrence is one multiply-add
 		if (active && k == 0)
 		{
 			const float* L = lred + 0 * rstride + s0;
@@ -622,7 +626,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 					Ln = vload<V>(L + (q + 1) * S);
 					In = vload<V>(inv + (q + 1) * S);
 				}
-				Rp = (Rq - Lq * Rp) * Iq;
+				Rp = Rq * Iq - Lq * Rp;
 				r[q * lpg] = Rp;
 			}
 			vec u = Rp;
@@ -697,12 +701,13 @@ __global__ __launch_bounds__(max_group_threads(V)) void sweep_partitioned_kernel
 }
 
 // Many microenvironments in one launch: workgroups walk a work list of (instance, group range) items;
-// every instance runs one substrate per thread with 16-element register blocks, and its workgroup shape
-// fits the launch's block size.
+// every instance runs one substrate per thread with 16-element register blocks (any substrate count), and
+// its workgroup shape fits the launch's block size.  (Splitting the batch by substrates-per-thread into
+// separate launches measured no faster.)
 template <int AXIS, int K>
 __global__ __launch_bounds__(batch_max_threads) void sweep_batched_kernel(const part_args* __restrict__ inst,
-																			const batch_item* __restrict__ items,
-																			int nitems)
+																		 const batch_item* __restrict__ items,
+																		 int nitems)
 {
 	extern __shared__ float lds[];
 	for (int it = blockIdx.x; it < nitems; it += gridDim.x)
@@ -711,6 +716,372 @@ __global__ __launch_bounds__(batch_max_threads) void sweep_batched_kernel(const 
 		const part_args a = inst[w.instance];
 		sweep_groups<batch_chunk, AXIS, 1, K>(a, lds, w.begin, w.end, 1);
 	}
+}
+
+// ------------------------------------------------------------------------------------------------
+// LDS-staged sweep
+// ------------------------------------------------------------------------------------------------
+//
+// Tiles of whole lines move between HBM and LDS as raw bytes with 16-byte accesses wherever the addresses
+// allow (any substrate count, any storage form); the partitioned algorithm then runs with one line per
+// thread (16-element register blocks) reading and writing the tile in LDS.  Tiles of one axis:
+//   x: `span` consecutive rows (z, y), span*X*S contiguous elements; line (row r, s), element i at
+//      r*XS + i*S + s;
+//   y: `span` consecutive columns q = x*S + s of one z plane, over all y: Y segments of span elements,
+//      XS apart in HBM; element i of line q at segment i;
+//   z: `span` consecutive positions p of the (y, q) plane, over all z: Z segments, Y*XS apart.
+// Each LDS segment starts at the same offset modulo 16 bytes as its HBM source, so the vector part of the
+// copy is aligned on both sides; the few elements before the first and after the last 16-byte boundary of a
+// segment move one by one.
+
+struct tile_args
+{
+	part_args a;	 // partition and coefficient tables of the axis (its lines/groups fields are unused)
+	int span;		 // x: rows per tile; y/z: columns per tile
+	int64_t tiles;	 // tiles of the axis
+	int pass_lines;	 // lines solved together (threads = pass_lines * P, rounded up to a wavefront)
+	int pitch;		 // y/z: LDS elements per segment (a multiple of 16 bytes, >= span + one vector)
+	int64_t tile_elems; // LDS elements of the tile buffer
+};
+
+struct tile_view
+{
+	int64_t g0;		 // HBM element offset of segment 0
+	int64_t gstride; // HBM elements between segments
+	int nseg, seglen;
+	int lines;
+	int64_t first; // x: first row; y: z plane; z: first plane position
+	int64_t q0;	   // y: first column
+};
+
+template <int AXIS>
+__device__ __forceinline__ tile_view tile_at(const tile_args& t, int64_t tile)
+{
+	const sweep_geometry& g = t.a.g;
+	const int64_t XS = (int64_t)g.X * g.S;
+	tile_view v;
+	if constexpr (AXIS == 0)
+	{
+		const int64_t rows = (int64_t)g.Y * g.Z;
+		v.first = tile * t.span;
+		const int nrows = (int)min((int64_t)t.span, rows - v.first);
+		v.g0 = v.first * XS;
+		v.gstride = 0;
+		v.nseg = 1;
+		v.seglen = (int)(nrows * XS);
+		v.lines = nrows * g.S;
+		v.q0 = 0;
+	}
+	else if constexpr (AXIS == 1)
+	{
+		const int64_t per_plane = (XS + t.span - 1) / t.span;
+		v.first = tile / per_plane;
+		v.q0 = (tile % per_plane) * t.span;
+		v.seglen = (int)min((int64_t)t.span, XS - v.q0);
+		v.g0 = v.first * (int64_t)g.Y * XS + v.q0;
+		v.gstride = XS;
+		v.nseg = g.Y;
+		v.lines = v.seglen;
+	}
+	else
+	{
+		const int64_t plane = (int64_t)g.Y * XS;
+		v.first = tile * t.span;
+		v.q0 = 0;
+		v.seglen = (int)min((int64_t)t.span, plane - v.first);
+		v.g0 = v.first;
+		v.gstride = plane;
+		v.nseg = g.Z;
+		v.lines = v.seglen;
+	}
+	return v;
+}
+
+// LDS element index of segment `sg`'s first element: segments are `pitch` apart (x: one segment at 0),
+// shifted by the HBM element offset modulo one 16-byte vector.
+template <int AXIS, int VE>
+__device__ __forceinline__ int lds_seg(const tile_args& t, const tile_view& v, int sg)
+{
+	const int shift = (int)((v.g0 + (int64_t)sg * v.gstride) & (VE - 1));
+	return (AXIS == 0 ? 0 : sg * t.pitch) + shift;
+}
+
+// HBM <-> LDS copy of a tile's raw elements (to_lds: load; else store).
+template <int AXIS, int K, bool to_lds>
+__device__ __forceinline__ void tile_copy(const tile_args& t, const tile_view& v, typename store_of<K>::elem* tl)
+{
+	using elem = typename store_of<K>::elem;
+	constexpr int VE = 16 / (int)sizeof(elem); // elements per 16-byte vector
+	elem* gd = (elem*)t.a.g.data;
+	const int maxvec = v.seglen / VE + 1;
+	const int64_t nslots = (int64_t)v.nseg * maxvec;
+	for (int64_t slot = threadIdx.x; slot < nslots; slot += blockDim.x)
+	{
+		const int sg = (int)(slot / maxvec);
+		const int vi = (int)(slot % maxvec);
+		const int64_t gs = v.g0 + (int64_t)sg * v.gstride;
+		const int head = (int)((VE - (gs & (VE - 1))) & (VE - 1));
+		if (head + (vi + 1) * VE > v.seglen)
+			continue;
+		const int e = head + vi * VE;
+		uint4* gp = (uint4*)(gd + gs + e);
+		uint4* lp = (uint4*)(tl + lds_seg<AXIS, VE>(t, v, sg) + e);
+		if constexpr (to_lds)
+			*lp = *gp;
+		else
+			*gp = *lp;
+	}
+	// elements before the first and after the last 16-byte boundary of each segment
+	const int64_t nscalar = (int64_t)v.nseg * 2 * (VE - 1);
+	for (int64_t slot = threadIdx.x; slot < nscalar; slot += blockDim.x)
+	{
+		const int sg = (int)(slot / (2 * (VE - 1)));
+		const int r = (int)(slot % (2 * (VE - 1)));
+		const int64_t gs = v.g0 + (int64_t)sg * v.gstride;
+		const int head = (int)min((int64_t)((VE - (gs & (VE - 1))) & (VE - 1)), (int64_t)v.seglen);
+		const int body = (v.seglen - head) / VE * VE;
+		int e;
+		if (r < VE - 1)
+		{
+			if (r >= head)
+				continue;
+			e = r;
+		}
+		else
+		{
+			e = head + body + (r - (VE - 1));
+			if (e >= v.seglen)
+				continue;
+		}
+		elem* gp = gd + gs + e;
+		elem* lp = tl + lds_seg<AXIS, VE>(t, v, sg) + e;
+		if constexpr (to_lds)
+			*lp = *gp;
+		else
+			*gp = *lp;
+	}
+}
+
+template <int AXIS, int K>
+__device__ __forceinline__ void sweep_tiles(const tile_args& t, float* lds, int64_t first_tile, int64_t end_tile,
+											int64_t step)
+{
+#pragma clang fp contract(fast)
+	using elem = typename store_of<K>::elem;
+	constexpr int VE = 16 / (int)sizeof(elem);
+	constexpr int M = batch_chunk;
+	const part_args& a = t.a;
+	const sweep_geometry& g = a.g;
+	const int S = g.S;
+	const int P = a.P;
+	const int lpp = t.pass_lines;
+	const int ncoef = 4 * a.ntypes * a.mb * S;
+	const int nred = 3 * 2 * P * S;
+	float* lcoef = lds;
+	float* lred = lds + ncoef;
+	float* rv = lred + nred; // [2P][lpp]
+	// tile buffer after the tables, 16-byte aligned
+	const int off = ((ncoef + nred + 2 * P * lpp) + 3) & ~3;
+	elem* tl = (elem*)(lds + off);
+
+	for (int i = threadIdx.x; i < ncoef; i += blockDim.x)
+		lcoef[i] = a.coef[i];
+	for (int i = threadIdx.x; i < nred; i += blockDim.x)
+		lred[i] = a.red[i];
+
+	// thread -> (line in the pass, block k); consecutive lanes take consecutive lines
+	const int line_local = threadIdx.x % lpp;
+	int k = threadIdx.x / lpp;
+	if (k >= P)
+		k = -1;
+	int start = 0, mk = 0, type = 0;
+	if (k >= 0)
+	{
+		start = k * a.base + min(k, a.rem);
+		mk = a.base + (k < a.rem ? 1 : 0);
+		type = (P == 1) ? 0 : (k == 0 ? 0 : (k == P - 1 ? 3 : (k < a.rem ? 1 : 2)));
+	}
+	const size_t fstride = (size_t)a.ntypes * a.mb * S;
+	const size_t rstride = (size_t)2 * P * S;
+	const uint32_t key = K == store_f32 ? 0u : rounding_key(g.rounding, 1 + AXIS);
+	const int grid[3] = { g.X, g.Y, g.Z };
+	const int64_t XS = (int64_t)g.X * S;
+	__syncthreads();
+
+	for (int64_t tile = first_tile; tile < end_tile; tile += step)
+	{
+		const tile_view v = tile_at<AXIS>(t, tile);
+		tile_copy<AXIS, K, true>(t, v, tl);
+		__syncthreads();
+		for (int l0 = 0; l0 < v.lines; l0 += lpp)
+		{
+			const int l = l0 + line_local;
+			const bool active = k >= 0 && l < v.lines;
+			// the line: its substrate, voxel coordinates and LDS addressing
+			int s = 0, lbase = 0, lstride = 0;
+			int coord[3] = { 0, 0, 0 };
+			int64_t gbase = 0, gstep = 0; // HBM element index of element i: gbase + i * gstep
+			if (active)
+			{
+				if constexpr (AXIS == 0)
+				{
+					const int r = l / S;
+					s = l - r * S;
+					const int64_t row = v.first + r;
+					coord[1] = (int)(row % g.Y);
+					coord[2] = (int)(row / g.Y);
+					lbase = lds_seg<AXIS, VE>(t, v, 0) + r * (int)XS + s;
+					lstride = S;
+					gbase = v.g0 + (int64_t)r * XS + s;
+					gstep = S;
+				}
+				else
+				{
+					const int64_t q = (AXIS == 1 ? v.q0 : v.first) + l; // column (y) or plane position (z)
+					s = (int)(q % S);
+					coord[0] = (int)((q / S) % g.X);
+					if constexpr (AXIS == 1)
+						coord[2] = (int)v.first;
+					else
+						coord[1] = (int)(q / XS);
+					lbase = l;
+					lstride = t.pitch;
+					gbase = v.g0 + l;
+					gstep = v.gstride;
+				}
+			}
+			// element i of this line in LDS (y/z: segment i is shifted by its HBM offset modulo a vector)
+			const int sh0 = (int)(v.g0 & (VE - 1)), shs = (int)(v.gstride & (VE - 1));
+			auto at = [&](int i) -> elem& {
+				if constexpr (AXIS == 0)
+					return tl[lbase + i * lstride];
+				else
+					return tl[lbase + i * lstride + ((sh0 + i * shs) & (VE - 1))];
+			};
+			const float* cr = lcoef + 0 * fstride + (size_t)type * a.mb * S + s;
+			const float* cg = lcoef + 1 * fstride + (size_t)type * a.mb * S + s;
+			const float* ca = lcoef + 2 * fstride + (size_t)type * a.mb * S + s;
+			const float* cgp = lcoef + 3 * fstride + (size_t)type * a.mb * S + s;
+
+			float x[M];
+			if (active)
+			{
+				const float inv = K == store_f32 ? 1.f : g.inv[s];
+#pragma unroll
+				for (int j = 0; j < M; j++)
+					if (j < mk)
+						x[j] = (float)at(start + j) * inv;
+				const float As = a.A[s];
+#pragma unroll
+				for (int j = 0; j < M; j++)
+				{
+					if (j < mk)
+					{
+						const float r = cr[j * S];
+						if (j < 2)
+							x[j] = r * x[j];
+						else
+							x[j] = r * x[j] - (r * As) * x[j - 1];
+					}
+				}
+#pragma unroll
+				for (int j = M - 3; j >= 1; j--)
+					if (j <= mk - 3)
+						x[j] = x[j] - cg[j * S] * x[j + 1];
+				if (mk >= 3)
+					x[0] = ca[0] * (x[0] - cg[0] * x[1]);
+				rv[(2 * k) * lpp + line_local] = x[0];
+#pragma unroll
+				for (int j = 1; j < M; j++)
+					if (j == mk - 1)
+						rv[(2 * k + 1) * lpp + line_local] = x[j];
+			}
+			__syncthreads();
+			if (active && k == 0)
+			{
+				const float* L = lred + 0 * rstride + s;
+				const float* inv = lred + 1 * rstride + s;
+				const float* cp = lred + 2 * rstride + s;
+				float* r = rv + line_local;
+				const int nq = 2 * P;
+				float Rp = r[0];
+				float Rn = r[lpp], Ln = L[S], In = inv[S];
+#pragma unroll 1
+				for (int q = 1; q < nq; q++)
+				{
+					const float Rq = Rn, Lq = Ln, Iq = In;
+					if (q + 1 < nq)
+					{
+						Rn = r[(q + 1) * lpp];
+						Ln = L[(q + 1) * S];
+						In = inv[(q + 1) * S];
+					}
+					Rp = Rq * Iq - Lq * Rp;
+					r[q * lpp] = Rp;
+				}
+				float u = Rp;
+				float Rb = r[(nq - 2) * lpp], Cb = cp[(nq - 2) * S];
+#pragma unroll 1
+				for (int q = nq - 2; q >= 0; q--)
+				{
+					const float Rq = Rb, Cq = Cb;
+					if (q > 0)
+					{
+						Rb = r[(q - 1) * lpp];
+						Cb = cp[(q - 1) * S];
+					}
+					u = Rq - Cq * u;
+					r[q * lpp] = u;
+				}
+			}
+			__syncthreads();
+			if (active)
+			{
+				const float u0 = rv[(2 * k) * lpp + line_local];
+				const float u1 = rv[(2 * k + 1) * lpp + line_local];
+				face_epilogue fe;
+				fe.init(g.fval, g.fcond, S, s, AXIS, coord, grid, g.dims);
+				const float sc = K == store_f32 ? 1.f : g.scale[s];
+#pragma unroll
+				for (int j = 0; j < M; j++)
+				{
+					if (j < mk)
+					{
+						float y;
+						if (j == 0)
+							y = u0;
+						else if (j == mk - 1)
+							y = u1;
+						else
+							y = x[j] - ca[j * S] * u0 - cgp[j * S] * u1;
+						const int i = start + j;
+						y = fe.apply(y, i, a.n);
+						if constexpr (K == store_f32)
+							at(i) = y;
+						else
+						{
+							// stochastic rounding keyed by the HBM element index (independent of the tiling)
+							const int64_t gi = gbase + (int64_t)i * gstep;
+							constexpr uint32_t drop = K == store_bf16 ? 0xFFFFu : 0x1FFFu;
+							const uint32_t rb = mix32((uint32_t)gi ^ ((uint32_t)((uint64_t)gi >> 32) * 0x9E3779B9u) ^ key);
+							const uint32_t b = (__float_as_uint(y * sc) + (rb & drop)) & ~drop;
+							at(i) = (elem)__uint_as_float(b);
+						}
+					}
+				}
+			}
+			__syncthreads();
+		}
+		tile_copy<AXIS, K, false>(t, v, tl);
+		__syncthreads();
+	}
+}
+
+template <int AXIS, int K>
+__global__ __launch_bounds__(1024) void sweep_staged_kernel(tile_args t)
+{
+	extern __shared__ float lds[];
+	sweep_tiles<AXIS, K>(t, lds, blockIdx.x, t.tiles, gridDim.x);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -792,7 +1163,7 @@ static size_t partition_lds_bytes(const partition& p, int S, int lpg)
 		   ((size_t)4 * p.ntypes * p.mb * S + (size_t)6 * p.P * S + (size_t)2 * p.P * lpg * p.vec);
 }
 
-partition choose_partition(const bfvm_ctx* ctx, int axis, int chunk, int lines_request, bool allow_vec)
+partition choose_partition(const bfvm_ctx* ctx, int axis, int chunk, int lines_request, int max_vec)
 {
 	const int n = ctx->mesh.grid[axis];
 	const int S = ctx->mesh.S;
@@ -801,12 +1172,14 @@ partition choose_partition(const bfvm_ctx* ctx, int axis, int chunk, int lines_r
 	if (n <= 1)
 		return none;
 	// 4 substrates per thread when they pack into 16-byte vectors (register chunk 16: 64 data VGPRs)
-	const bool vec4 = allow_vec && S % 4 == 0;
+	const bool vec4 = max_vec >= 4 && S % 4 == 0;
+	auto vec_for = [&](int M) { return (vec4 && M == 16) ? 4 : 1; };
 	// candidate register chunks, smallest first from the request; 16-bit storage instantiates 16 and 32 only
 	const bool wide = ctx->storage != BFVM_STORAGE_F32;
 	std::vector<int> candidates;
+	const int first = chunk > 0 ? chunk : (vec4 ? 16 : 32);
 	for (int c : chunk_sizes)
-		if (c >= supported_chunk(chunk > 0 ? chunk : (vec4 ? 16 : 32)) && (!wide || c == 16 || c == 32))
+		if (c >= supported_chunk(first) && (!wide || c == 16 || c == 32))
 			candidates.push_back(c);
 	if (candidates.empty())
 		candidates.push_back(32);
@@ -817,7 +1190,7 @@ partition choose_partition(const bfvm_ctx* ctx, int axis, int chunk, int lines_r
 		partition p;
 		p.n = n;
 		p.chunk = M;
-		p.vec = (vec4 && M == 16) ? 4 : 1;
+		p.vec = vec_for(M);
 		p.P = (n + M - 1) / M;
 		p.base = n / p.P;
 		p.rem = n % p.P;
@@ -849,7 +1222,9 @@ void build_partition(bfvm_ctx* ctx, int axis, int chunk, int lines_request)
 {
 	auto& t = ctx->axes[axis];
 	free_partition_tables(t);
-	t.part = choose_partition(ctx, axis, chunk, lines_request, true);
+	// the staged sweep runs one line per thread on 16-element blocks
+	const bool staged = ctx->algorithm == BFVM_SWEEP_STAGED;
+	t.part = choose_partition(ctx, axis, staged ? batch_chunk : chunk, lines_request, staged ? 1 : 4);
 	if (t.part.P > 0)
 		alloc_partition_tables(t.part, ctx->mesh.S, t.coef, t.red, t.A);
 }
@@ -872,6 +1247,12 @@ void launch_partition_coefficients(bfvm_ctx* ctx, int axis)
 	launch_partition_tables(ctx, axis, t.part, t.coef, t.red, t.A, ctx->stream);
 }
 
+// Workgroups of a single-instance sweep; each loops over groups when there are more.
+#ifndef BFVM_SWEEP_MAX_BLOCKS
+#define BFVM_SWEEP_MAX_BLOCKS 8192
+#endif
+static constexpr int64_t sweep_max_blocks = BFVM_SWEEP_MAX_BLOCKS;
+
 static part_args make_part_args(const sweep_geometry& g, const partition& p, const float* coef, const float* red,
 								const float* A)
 {
@@ -888,6 +1269,7 @@ static part_args make_part_args(const sweep_geometry& g, const partition& p, con
 	a.A = A;
 	const int64_t total = (int64_t)g.X * g.Y * g.Z * g.S;
 	a.lines = total / ((int64_t)p.n * p.vec);
+	a.vec = p.vec;
 	a.lines_per_group = p.lpg;
 	a.groups = (a.lines + p.lpg - 1) / p.lpg;
 	return a;
@@ -900,13 +1282,12 @@ static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const a
 	const part_args a = make_part_args(g, p, t.coef, t.red, t.A);
 	const int threads = p.threads;
 	const size_t lds = partition_lds_bytes(p, g.S, p.lpg);
-	const int64_t max_blocks = 8192;
-	const int blocks = (int)std::min<int64_t>(a.groups, max_blocks);
+	const int blocks = (int)std::min<int64_t>(a.groups, sweep_max_blocks);
 	hipLaunchKernelGGL((sweep_partitioned_kernel<M, AXIS, V, K>), dim3(blocks), dim3(threads), lds, ctx->stream, a);
 	BFVM_HIP(hipGetLastError());
 }
 
-// FP32 storage: every register chunk; 16-bit storage: chunks 16 and 32 (build_partition restricts them)
+// FP32 storage: every register chunk; 16-bit storage: chunks 16 and 32 (choose_partition restricts them)
 template <int AXIS, int K>
 static void launch_partitioned_axis(bfvm_ctx* ctx, const sweep_geometry& g, const axis_tables& t)
 {
@@ -958,6 +1339,73 @@ static void launch_exact_axis(bfvm_ctx* ctx, const sweep_geometry& g, const axis
 	BFVM_HIP(hipGetLastError());
 }
 
+// A workgroup's LDS holds the coefficient tables, the reduced rows of one pass and the tile; the whole
+// stays within 64 KB (tiles of at most 48 KB); one pass solves up to 1024 / P lines.
+static constexpr size_t staged_lds_limit = 64 * 1024;
+static constexpr size_t staged_tile_bytes = 48 * 1024;
+
+static tile_args make_tile_args(const bfvm_ctx* ctx, int axis, const sweep_geometry& g, const partition& p,
+								const float* coef, const float* red, const float* A)
+{
+	tile_args t;
+	t.a = make_part_args(g, p, coef, red, A);
+	const size_t e = storage_bytes(ctx->storage);
+	const int VE = (int)(16 / e);
+	const int64_t XS = (int64_t)g.X * g.S;
+	const int n = p.n;
+	const int max_lines = std::max(1, 1024 / p.P);
+	const size_t tables = sizeof(float) * ((size_t)4 * p.ntypes * p.mb * g.S + (size_t)6 * p.P * g.S +
+										   (size_t)2 * p.P * max_lines + 4);
+	if (tables + 16 * n * e > staged_lds_limit)
+		throw std::runtime_error("staged sweep: coefficient tables leave no room for a tile");
+	const size_t budget = std::min(staged_tile_bytes, staged_lds_limit - tables);
+	if (axis == 0)
+	{
+		const int64_t rows = (int64_t)g.Y * g.Z;
+		t.span = (int)std::max<int64_t>(1, std::min<int64_t>(rows, (budget - 16) / (XS * e)));
+		t.tiles = (rows + t.span - 1) / t.span;
+		t.pitch = 0;
+		t.tile_elems = t.span * XS + VE;
+		t.pass_lines = (int)std::min<int64_t>(t.span * (int64_t)g.S, max_lines);
+	}
+	else
+	{
+		const int64_t width = axis == 1 ? XS : (int64_t)g.Y * XS; // columns of a plane / plane positions
+		// pitch = span rounded up to a vector, plus one vector of shift room
+		int span = (int)(budget / ((size_t)n * e)) / VE * VE - VE;
+		span = std::max(VE, std::min(span, 256));
+		t.span = (int)std::min<int64_t>(span, (width + VE - 1) / VE * VE);
+		t.pitch = (t.span + VE - 1) / VE * VE + VE;
+		t.tiles = (axis == 1 ? (int64_t)g.Z : 1) * ((width + t.span - 1) / t.span);
+		t.tile_elems = (int64_t)n * t.pitch;
+		t.pass_lines = std::min(t.span, max_lines);
+	}
+	return t;
+}
+
+static size_t tile_lds_bytes(const tile_args& t, int S, int storage)
+{
+	const int P = t.a.P;
+	const size_t tables = (size_t)4 * t.a.ntypes * t.a.mb * S + (size_t)6 * P * S + (size_t)2 * P * t.pass_lines;
+	return ((tables + 3) & ~(size_t)3) * sizeof(float) + (size_t)t.tile_elems * storage_bytes(storage);
+}
+
+static int tile_threads(const tile_args& t) { return (t.pass_lines * t.a.P + 63) / 64 * 64; }
+
+template <int AXIS, int K>
+static void launch_staged_axis(bfvm_ctx* ctx, const sweep_geometry& g, const axis_tables& t)
+{
+	if (t.part.chunk != batch_chunk)
+		throw std::logic_error("staged sweep needs the 16-element partition");
+	const tile_args ta = make_tile_args(ctx, AXIS, g, t.part, t.coef, t.red, t.A);
+	const size_t lds = tile_lds_bytes(ta, g.S, ctx->storage);
+	if (lds > staged_lds_limit)
+		throw std::runtime_error("staged sweep: tile does not fit LDS");
+	const int blocks = (int)std::min<int64_t>(ta.tiles, 4096);
+	hipLaunchKernelGGL((sweep_staged_kernel<AXIS, K>), dim3(blocks), dim3(tile_threads(ta)), lds, ctx->stream, ta);
+	BFVM_HIP(hipGetLastError());
+}
+
 template <int K>
 static void launch_sweep_storage(bfvm_ctx* ctx, int axis)
 {
@@ -982,6 +1430,16 @@ static void launch_sweep_storage(bfvm_ctx* ctx, int axis)
 			launch_exact_axis<1>(ctx, g, t);
 		else
 			launch_exact_axis<2>(ctx, g, t);
+		return;
+	}
+	if (ctx->algorithm == BFVM_SWEEP_STAGED)
+	{
+		if (axis == 0)
+			launch_staged_axis<0, K>(ctx, g, t);
+		else if (axis == 1)
+			launch_staged_axis<1, K>(ctx, g, t);
+		else
+			launch_staged_axis<2, K>(ctx, g, t);
 		return;
 	}
 	if (axis == 0)
@@ -1053,7 +1511,7 @@ void prepare_batch_sweep(bfvm_batch* b, int axis)
 		const int P = (n + batch_chunk - 1) / batch_chunk;
 		const int request = axis == 0 ? std::max(1, batch_group_threads / (S * P))
 									  : std::max(8, batch_group_threads / P / 8 * 8);
-		const partition p = choose_partition(ctx, axis, batch_chunk, request, false);
+		const partition p = choose_partition(ctx, axis, batch_chunk, request, 1);
 		if (p.chunk != batch_chunk || p.vec != 1 || p.threads > batch_max_threads)
 			throw std::runtime_error("batched sweep: instance " + std::to_string(i) +
 									 " does not fit one workgroup of 16-element blocks (axis too long)");

@@ -280,9 +280,10 @@ __device__ __forceinline__ void gradient_tile(const gradient_args& a, int tile)
 			gz = (c - zm) * bz;
 		else
 			gz = (zp - zm) * cz;
-		vstore<V>(g0 + o, gx);
-		vstore<V>(g1 + o, gy);
-		vstore<V>(g2 + o, gz);
+		// streaming outputs: non-temporal stores keep them from displacing the density rows in L2
+		__builtin_nontemporal_store(gx, (vec*)(g0 + o));
+		__builtin_nontemporal_store(gy, (vec*)(g1 + o));
+		__builtin_nontemporal_store(gz, (vec*)(g2 + o));
 		zm = c;
 		c = zp;
 	}
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(grad_tile) void gradient_batched_kernel(const gradi
 
 // Calibration kernel: float4 copy of n4 16-byte elements.  Its bandwidth is the practical ceiling the
 // solver kernels are compared against, and its counter readings scale the traffic counters.
-__global__ __launch_bounds__(256) void copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src,
+__global__ __launch_bounds__(1024) void copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src,
 												   int64_t n4)
 {
 	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -333,7 +334,10 @@ void launch_copy(bfvm_ctx* ctx)
 	const int64_t n4 = bytes / 16;
 	if (n4 == 0)
 		return;
-	hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, ctx->stream, (float4*)ctx->grad,
+	// 1024-thread workgroups, up to 65536 of them (about two 16-byte elements per thread at 2 GB): the launch
+	// shape that moves the most bytes per second on MI355X
+	const int64_t blocks = std::min<int64_t>(65536, (n4 + 1023) / 1024);
+	hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(1024), 0, ctx->stream, (float4*)ctx->grad,
 					   (const float4*)ctx->dens, n4);
 	BFVM_HIP(hipGetLastError());
 }
@@ -393,8 +397,9 @@ void prepare_batch_gradient(bfvm_batch* b)
 			args[i] = gradient_args {};
 			continue;
 		}
-		const int tpr = (ctx->mesh.grid[0] * ctx->mesh.S / (ctx->mesh.S % 4 == 0 ? 4 : 1) + grad_tile - 1) / grad_tile;
-		const int64_t row_tiles = (int64_t)ctx->mesh.grid[1] * tpr;
+		// tiles of 16 z planes: ask for (row tiles) x ceil(Z / 16)
+		const gradient_args probe = make_gradient_args(ctx, 1);
+		const int64_t row_tiles = (int64_t)probe.ntiles;
 		args[i] = make_gradient_args(ctx, row_tiles * ((ctx->mesh.grid[2] + 15) / 16));
 		for (int64_t t0 = 0; t0 < args[i].ntiles; t0 += 4)
 			items.push_back(batch_item { (int32_t)i, 0, t0, std::min<int64_t>(args[i].ntiles, t0 + 4) });

@@ -218,6 +218,27 @@ def test_gpu_partitioned_diffusion(name, dims, shape, S, chunk, pkg, oracle_mod)
     assert_close(g.densities, o.densities, what=f"chunk {chunk} partition {[g.partition(a) for a in range(dims)]}")
 
 
+STAGED_SHAPES = SHAPES + [
+    ("3d_odd_s", 3, (37, 23, 19), 3),    # rows not 16-byte aligned: head/tail elements of every segment
+    ("3d_s5_long", 3, (21, 130, 9), 5),  # long y lines, misaligned segments
+    ("3d_s7", 3, (50, 31, 41), 7),
+]
+
+
+@pytest.mark.parametrize("name,dims,shape,S", STAGED_SHAPES, ids=[s[0] for s in STAGED_SHAPES])
+def test_gpu_staged_diffusion(name, dims, shape, S, pkg, oracle_mod):
+    """The partitioned algorithm on LDS-staged tiles (any substrate count, unaligned rows), Dirichlet faces
+    and interior voxels included."""
+    p = random_problem(dims, shape, S, seed=43)
+    o, g = build_pair(pkg, oracle_mod, p, algorithm=pkg.BFVM_SWEEP_STAGED, chunk=0)
+    g.store()
+    for _ in range(3):
+        o.diffusion()
+        g.diffusion()
+    g.load()
+    assert_close(g.densities, o.densities, what=f"staged partition {[g.partition(a) for a in range(dims)]}")
+
+
 @pytest.mark.parametrize("name,dims,shape,S", [s for s in SHAPES if min(s[2][:s[1]]) >= 2],
                          ids=[s[0] for s in SHAPES if min(s[2][:s[1]]) >= 2])
 def test_gpu_gradient_bitwise(name, dims, shape, S, pkg, oracle_mod):

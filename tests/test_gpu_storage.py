@@ -169,7 +169,7 @@ def rel_errors(got, exp, S):
     return l2, mx
 
 
-def run_precision(pkg, oracle_mod, cfg, storage, checkpoints):
+def run_precision(pkg, oracle_mod, cfg, storage, checkpoints, algorithm=None):
     from parabiofvm_amd.configs import apply_config
 
     args = (cfg["dims"], cfg["mins"], cfg["maxs"], cfg["voxel_shape"], cfg["S"], cfg["dt"], cfg["D"], cfg["decay"],
@@ -180,6 +180,8 @@ def run_precision(pkg, oracle_mod, cfg, storage, checkpoints):
     for be in (o, g):
         apply_config(cfg, be)
         be.initialize()
+    if algorithm is not None:
+        g.set_sweep_algorithm(algorithm, 0)
     g.store()
     S = cfg["S"]
     rows = []
@@ -206,13 +208,15 @@ def check_rows(rows, storage):
         assert max(r["densities_l2"]) <= bound, r
 
 
+@pytest.mark.parametrize("algorithm", ["partitioned", "staged"])
 @pytest.mark.parametrize("storage", ["bf16", "f16", "f32"])
-def test_precision_small_clustered_dirichlet(storage, pkg, oracle_mod):
+def test_precision_small_clustered_dirichlet(storage, algorithm, pkg, oracle_mod):
     """Clustered agents with Dirichlet faces and interior voxels (a 40^3 x 8 version of the large instance)."""
     from parabiofvm_amd.configs import make_config
 
     cfg = make_config(1, seed=21, shape=(40, 40, 40), agents=4000)
-    check_rows(run_precision(pkg, oracle_mod, cfg, storage, [1, 20]), storage)
+    alg = pkg.BFVM_SWEEP_STAGED if algorithm == "staged" else pkg.BFVM_SWEEP_PARTITIONED
+    check_rows(run_precision(pkg, oracle_mod, cfg, storage, [1, 20], alg), storage)
 
 
 @pytest.mark.parametrize("storage", ["bf16", "f16", "f32"])

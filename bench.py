@@ -175,6 +175,103 @@ def bench_batch(args):
     print(json.dumps(result), flush=True)
 
 
+def rel_errors_by_substrate(got, ref, S, rows=1 << 24):
+    """Per-substrate relative L2 and max relative error of got vs ref ([V][S] float32), in row chunks."""
+    num = np.zeros(S)
+    den = np.zeros(S)
+    mx = np.zeros(S)
+    g2 = got.reshape(-1, S)
+    r2 = ref.reshape(-1, S)
+    for i in range(0, g2.shape[0], rows):
+        g = g2[i:i + rows].astype(np.float64)
+        r = r2[i:i + rows].astype(np.float64)
+        d = g - r
+        num += (d * d).sum(axis=0)
+        den += (r * r).sum(axis=0)
+        mx = np.maximum(mx, (np.abs(d) / np.maximum(np.abs(r), 1e-6)).max(axis=0))
+    return np.sqrt(num / np.maximum(den, 1e-300)), mx
+
+
+def bench_stress(args):
+    """640^3 x 16 substrates in 16-bit storage with 1e7 clustered agents, against an FP32-storage run of the
+    same workload on the same GPU; per-kernel bandwidth of the 16-bit run and both secretion reductions."""
+    import parabiofvm_amd
+    from parabiofvm_amd.configs import apply_config, make_config
+
+    storage = args.storage if args.storage != "f32" else "bf16"
+    t = time.perf_counter()
+    cfg = make_config(4, shape=args.shape, agents=args.agents)
+    X, Y, Z = cfg["grid"]
+    S = cfg["S"]
+    VS = cfg["updates_per_step"]
+    A = int(cfg["agents"]["volumes"].shape[0])
+    log(f"[bench] stress workload generated in {time.perf_counter() - t:.1f}s: {X}x{Y}x{Z} x {S}, {A} agents")
+
+    def make(st):
+        g = parabiofvm_amd.DeviceSolver(cfg["dims"], cfg["mins"], cfg["maxs"], cfg["voxel_shape"], S, cfg["dt"],
+                                        cfg["D"], cfg["decay"], np.zeros(S, dtype=np.float32), device=0)
+        apply_config(cfg, g)
+        g.set_storage(st)
+        g.initialize()
+        g.store()
+        log(f"[bench] {st} context initialized and stored")
+        return g
+
+    lo = make(storage)
+    for i in range(args.warmup):
+        lo.timestep(i == 0, True)
+    lo.wait()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        lo.timestep(i % 10 == 0, True)
+    lo.wait()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    log(f"[bench] {storage}: {args.steps} steps in {elapsed:.3f}s -> {ms_per_step:.3f} ms/step")
+    # the FP32-storage run of the same steps (the agent terms do not change, so recomputing is idempotent)
+    ref = make("f32")
+    for i in range(args.warmup + args.steps):
+        ref.timestep(i % 10 == 0, True)
+    ref.wait()
+    lo.load_densities()
+    ref.load_densities()
+    l2, mx = rel_errors_by_substrate(lo.densities, ref.densities, S)
+    log(f"[bench] {storage} vs FP32 after {args.warmup + args.steps} steps: rel L2 max {l2.max():.2e}, "
+        f"max rel {mx.max():.2e}")
+    ref.close()
+    e = 2
+    per = {"sweep_x": 2 * e * VS, "sweep_y": 2 * e * VS, "sweep_z": 2 * e * VS, "gradient": (e + 12) * VS,
+           "copy": 2 * e * VS}
+    kernel_ms = {}
+    for k in ("sweep_x", "sweep_y", "sweep_z", "gradient", "copy", "secretion", "secretion_recompute"):
+        kernel_ms[k] = lo.benchmark(k, args.kernel_iters)
+        gbs = f", {per[k] / (kernel_ms[k] * 1e-3) / 1e9:.0f} GB/s" if k in per else ""
+        log(f"[bench] {storage} {k}: {kernel_ms[k]:.4f} ms{gbs}")
+    # the sorted (segmented) reduction at the same contention, for comparison with the claim-and-atomics one
+    lo.set_secretion_reduction("sorted")
+    lo.store_agents()
+    sorted_recompute = lo.benchmark("secretion_recompute", args.kernel_iters)
+    log(f"[bench] {storage} secretion_recompute with the sorted reduction: {sorted_recompute:.4f} ms")
+    result = {
+        "metric": METRIC, "value": VS * args.steps / elapsed, "unit": "voxel·substrate updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": (f"stress: {X}x{Y}x{Z} voxels x {S} substrates, {storage.upper()} storage, {A} agents "
+                                f"in 64 clusters, 6 Dirichlet faces + 0.5% interior Dirichlet voxels, gradient every "
+                                f"step"),
+                   "grid": [X, Y, Z], "substrates": S, "agents": A, "storage": storage},
+        "hbm_frac_step": round((6 * e + 12) * VS / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
+        "kernel_gbs": {k: round(per[k] / (kernel_ms[k] * 1e-3) / 1e9, 1) for k in per},
+        "secretion_recompute_sorted_ms": round(sorted_recompute, 4),
+        "error_vs_f32": {"steps": args.warmup + args.steps, "rel_l2": [float(v) for v in l2],
+                         "max_rel": [float(v) for v in mx]},
+        "roofline": None, "cpu_baseline": None,
+    }
+    lo.close()
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +289,7 @@ def main():
                     help="density storage in HBM between kernels (arithmetic is FP32 in every form)")
     ap.add_argument("--batch", action="store_true", help="the batched workload: many uneven microenvironments")
     ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--stress", action="store_true", help="the 640^3 x 16 substrate, 1e7 agent stress workload")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -201,6 +299,9 @@ def main():
         sys.exit(2)
     if args.batch:
         bench_batch(args)
+        return
+    if args.stress:
+        bench_stress(args)
         return
 
     import parabiofvm_amd

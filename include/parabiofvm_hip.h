@@ -157,6 +157,15 @@ int bfvm_initialize(bfvm_ctx* ctx, const bfvm_microenvironment_desc* desc);
  * FP32 and are converted on the device. */
 int bfvm_set_storage(bfvm_ctx* ctx, int32_t storage);
 
+/* bulk_solver (host/bulk_solver.cpp:23-35) on the device: per voxel and substrate
+ * D = (D + dt*S*T) / (1 + dt*(U + S)).  The reference evaluates three per-voxel callbacks at every solve
+ * (microenvironment.h:52 supply_rate_func, uptake_rate_func, supply_target_densities_func); here their
+ * values are uploaded as [V][S] fields in the densities' layout and kept until the next upload.  The FP32
+ * result equals the reference's bit for bit. */
+int bfvm_bulk_store_rates(bfvm_ctx* ctx, const float* supply_rates, const float* uptake_rates,
+						  const float* supply_target_densities);
+int bfvm_bulk_solve(bfvm_ctx* ctx);
+
 /* How agents sharing a voxel are combined (BFVM_REDUCTION_*), no counterpart in the reference, whose cell
  * solvers always claim-and-add (host/cell_solver.cpp:87-135).  The sorted form orders the agents by voxel
  * when they are uploaded (rocPRIM radix sort) and sums each voxel's run in agent order with the lowest agent

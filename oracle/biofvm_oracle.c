@@ -724,6 +724,24 @@ int OR_FN(or_gradient_solve)(or_state* st)
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* bulk supply / uptake — host/bulk_solver.cpp:23-35, 98-120                                  */
+/* ------------------------------------------------------------------------------------------ */
+
+/* The reference asks per-voxel callbacks for S, U and T at every solve; the oracle takes the values they
+ * return as [V][S] arrays (the densities' layout) and applies, per voxel and substrate,
+ *   D = (D + dt*S*T) / (1 + dt*(U + S))
+ * with the reference's operation order: ((dt * S) * T) in the numerator. */
+int OR_FN(or_bulk_solve)(or_state* st, const or_real* supply, const or_real* uptake, const or_real* target)
+{
+	const long long n = (long long)st->grid[0] * st->grid[1] * st->grid[2] * st->S;
+	const or_real dt = st->dt;
+#pragma omp parallel for schedule(static)
+	for (long long i = 0; i < n; i++)
+		st->dens[i] = (st->dens[i] + dt * supply[i] * target[i]) / (1 + dt * (uptake[i] + supply[i]));
+	return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* secretion / uptake — host/cell_solver.cpp:15-346                                           */
 /* ------------------------------------------------------------------------------------------ */
 

@@ -71,6 +71,8 @@ int OR_FN(or_diffusion_solve)(or_state* st);		   /* host/diffusion_solver.cpp:60
 int OR_FN(or_dirichlet_solve)(or_state* st);		   /* host/dirichlet_solver.cpp:112-144 */
 int OR_FN(or_gradient_solve)(or_state* st);		   /* host/gradient_solver.cpp:275-291 */
 int OR_FN(or_cell_simulate)(or_state* st, int recompute); /* host/cell_solver.cpp:272-299 */
+/* host/bulk_solver.cpp:23-35: supply, uptake, target are the callbacks' values as [V][S] arrays */
+int OR_FN(or_bulk_solve)(or_state* st, const or_real* supply, const or_real* uptake, const or_real* target);
 int OR_FN(or_cell_release)(or_state* st, int index);	   /* host/cell_solver.cpp:340-346 */
 
 /* One timestep in a single omp parallel region (main.cpp:64-107 pattern), in the order

@@ -57,6 +57,7 @@ def _lib(dtype):
         "dirichlet": fn("or_dirichlet_solve", ctypes.c_int, P),
         "gradient": fn("or_gradient_solve", ctypes.c_int, P),
         "cell": fn("or_cell_simulate", ctypes.c_int, P, ctypes.c_int),
+        "bulk": fn("or_bulk_solve", ctypes.c_int, P, P, P, P),
         "release": fn("or_cell_release", ctypes.c_int, P, ctypes.c_int),
         "timestep": fn("or_timestep", ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P),
         "voxel_positions": fn("or_voxel_positions", None, P, ctypes.c_int, P, P),
@@ -191,6 +192,11 @@ class Oracle:
 
     def cell(self, recompute):
         self._check(self.api["cell"](self.h, int(bool(recompute))))
+
+    def bulk(self, supply, uptake, target):
+        """host/bulk_solver.cpp: the callbacks' values as [V][S] arrays."""
+        arrs = [np.ascontiguousarray(a, dtype=self.dtype) for a in (supply, uptake, target)]
+        self._check(self.api["bulk"](self.h, *[_ptr(a) for a in arrs]))
 
     def release(self, index):
         self._check(self.api["release"](self.h, int(index)))

@@ -90,6 +90,8 @@ PROTOTYPES = {
     "bfvm_initialize": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MicroenvironmentDesc)]),
     "bfvm_set_storage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "bfvm_set_secretion_reduction": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "bfvm_bulk_store_rates": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "bfvm_bulk_solve": (ctypes.c_int, [ctypes.c_void_p]),
     "bfvm_set_sweep_algorithm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "bfvm_store_densities": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "bfvm_store_densities_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

@@ -39,6 +39,8 @@ def make_config(index, seed=1234, shape=None, substrates=None, agents=None, step
     index 0: 100x100x100, S=4, 10k uniform agents, no Dirichlet, 100 steps.
     index 1: 400x400x400, S=8, 1e6 agents in 16 Gaussian clusters (sigma 400), 6 Dirichlet faces + 1%
              interior Dirichlet voxels, 50 steps with the gradient.
+    index 4: 640x640x640, S=16, 1e7 agents in 64 Gaussian clusters (sigma 200), 6 Dirichlet faces + 0.5%
+             interior Dirichlet voxels, 20 steps with the gradient (the stress workload, 16-bit storage).
     `shape`, `substrates`, `agents`, `steps`, `dirichlet` override the sizes (parity tests use small ones).
     """
     dx = 20
@@ -48,6 +50,9 @@ def make_config(index, seed=1234, shape=None, substrates=None, agents=None, step
     elif index == 1:
         grid = (400, 400, 400)
         S, A, nsteps, dir_on = 8, 1_000_000, 50, True
+    elif index == 4:
+        grid = (640, 640, 640)
+        S, A, nsteps, dir_on = 16, 10_000_000, 20, True
     else:
         raise ValueError("unknown config")
     if shape is not None:
@@ -70,9 +75,12 @@ def make_config(index, seed=1234, shape=None, substrates=None, agents=None, step
 
     if index == 0:
         pos = rng.uniform(0.0, 1.0, size=(A, 3)) * np.asarray(maxs, dtype=np.float64)
-    else:
+    elif index == 1:
         sigma = 400.0 * (grid[0] / 400.0 if shape is not None else 1.0)
         pos = clustered_positions(rng, A, 16, max(sigma, dx), maxs)
+    else:
+        sigma = 200.0 * (grid[0] / 640.0 if shape is not None else 1.0)
+        pos = clustered_positions(rng, A, 64, max(sigma, dx), maxs)
     agents_arr = {
         "secretion_rates": rng.uniform(0.0, 1.0, size=A * S).astype(np.float32),
         "uptake_rates": rng.uniform(0.0, 1.0, size=A * S).astype(np.float32),
@@ -90,7 +98,7 @@ def make_config(index, seed=1234, shape=None, substrates=None, agents=None, step
         for d in range(3):
             for mx in (0, 1):
                 faces[(d, mx)] = (rng.uniform(0.0, 1.0, size=S).astype(np.float32), np.ones(S, dtype=np.uint8))
-        k = max(1, V // 100)
+        k = max(1, V // (200 if index == 4 else 100))
         lin = np.unique(rng.integers(0, V, size=k))
         x = lin % grid[0]
         y = (lin // grid[0]) % grid[1]

@@ -3,12 +3,13 @@
 using namespace biofvm;
 using namespace biofvm::solvers::device;
 
-solver::solver() : cell(ctx), diffusion(ctx), gradient(ctx) {}
+solver::solver() : bulk(ctx), cell(ctx), diffusion(ctx), gradient(ctx) {}
 
 void solver::initialize(microenvironment& m)
 {
 	once([&] {
 		ctx.initialize(m);
+		bulk.initialize(m); // host/solver.cpp:5-10 initializes bulk, cell, then diffusion
 		cell.initialize(m);
 		diffusion.initialize(m);
 	});

@@ -1,6 +1,7 @@
 // The device solver facade (interface of src/solver/device/CUDA/solver.h:12-31 of the reference).
 #pragma once
 
+#include "bulk_solver.h"
 #include "cell_solver.h"
 #include "device_context.h"
 #include "diffusion_solver.h"
@@ -19,6 +20,7 @@ public:
 
 	static constexpr bool is_device_solver = true;
 
+	bulk_solver bulk; // the host solver's member (host/solver.h:17), here on the device
 	cell_solver cell;
 	diffusion_solver diffusion;
 	gradient_solver gradient;

@@ -112,6 +112,8 @@ static void free_fields(bfvm_ctx* ctx)
 	dfree(ctx->dinv);
 	dfree(ctx->maxbits);
 	dfree(ctx->rounding_step);
+	dfree(ctx->bulk_num);
+	dfree(ctx->bulk_den);
 	dfree(ctx->staging);
 	ctx->staging_elems = 0;
 	dfree(ctx->grad);
@@ -449,6 +451,47 @@ int bfvm_set_storage(bfvm_ctx* ctx, int32_t storage)
 		if (storage != BFVM_STORAGE_F32 && storage != BFVM_STORAGE_BF16 && storage != BFVM_STORAGE_F16)
 			throw std::invalid_argument("unknown storage form");
 		ctx->storage_request = storage;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_bulk_store_rates(bfvm_ctx* ctx, const float* supply_rates, const float* uptake_rates,
+						  const float* supply_target_densities)
+{
+	return guarded("bfvm_bulk_store_rates", [&] {
+		require_init(ctx);
+		if (!supply_rates || !uptake_rates || !supply_target_densities)
+			throw std::invalid_argument("null rate field");
+		BFVM_HIP(hipSetDevice(ctx->device));
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		const size_t n = ctx->mesh.V * ctx->mesh.S;
+		const float dt = ctx->mesh.dt;
+		// host/bulk_solver.cpp:30-33 operation order: (D + dt*S*T) / (1 + dt*(U + S))
+		std::vector<float> num(n), den(n);
+		for (size_t i = 0; i < n; i++)
+		{
+			num[i] = dt * supply_rates[i] * supply_target_densities[i];
+			den[i] = 1 + dt * (uptake_rates[i] + supply_rates[i]);
+		}
+		if (!ctx->bulk_num)
+		{
+			dalloc(ctx->bulk_num, n);
+			dalloc(ctx->bulk_den, n);
+		}
+		BFVM_HIP(hipMemcpy(ctx->bulk_num, num.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+		BFVM_HIP(hipMemcpy(ctx->bulk_den, den.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_bulk_solve(bfvm_ctx* ctx)
+{
+	return guarded("bfvm_bulk_solve", [&] {
+		require_init(ctx);
+		if (!ctx->bulk_num)
+			throw std::logic_error("no bulk rates uploaded (call bfvm_bulk_store_rates first)");
+		BFVM_HIP(hipSetDevice(ctx->device));
+		launch_bulk(ctx);
 		return BFVM_OK;
 	});
 }

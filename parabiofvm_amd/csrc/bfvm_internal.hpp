@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <algorithm>
 #include <cstdio>
 #include <string>
 #include <type_traits>
@@ -296,6 +297,8 @@ struct bfvm_ctx
 	std::vector<float> bound_max; // [S] largest |Dirichlet value| with its condition set
 	unsigned* maxbits = nullptr;  // [S] device scratch of the FP16 scale choice
 	unsigned* rounding_step = nullptr; // device counter keying the stochastic rounding of 16-bit stores
+	float* bulk_num = nullptr; // [V*S] dt*S*T of the bulk supply/uptake fields (null until uploaded)
+	float* bulk_den = nullptr; // [V*S] 1 + dt*(U + S)
 	float* staging = nullptr;	  // FP32 staging for host <-> 16-bit conversions
 	size_t staging_elems = 0;
 	float* grad = nullptr;
@@ -381,6 +384,7 @@ void launch_max_abs(bfvm_ctx* ctx, const float* src, int64_t n, unsigned* out);
 void launch_advance_rounding(bfvm_ctx* ctx);
 
 void launch_cell(bfvm_ctx* ctx, bool recompute);
+void launch_bulk(bfvm_ctx* ctx);
 void launch_cell_voxels(bfvm_ctx* ctx);
 void launch_cell_sort(bfvm_ctx* ctx); // sorted reduction: order the agents by voxel (upload time)
 void launch_cell_release(bfvm_ctx* ctx, int index);

@@ -172,6 +172,10 @@ class DeviceSolver:
                 setattr(desc, k, _fp(a[k]))
         check("bfvm_store_agents", self.lib.bfvm_store_agents(self.h, ctypes.byref(desc)))
 
+    def load_densities(self):
+        """Densities only (the gradients of a large mesh need not cross to the host)."""
+        check("bfvm_load_densities", self.lib.bfvm_load_densities(self.h, self.densities.ctypes.data))
+
     def load(self):
         """load_data_from_solver: densities, gradients and internalized substrates to the host."""
         check("bfvm_load_densities", self.lib.bfvm_load_densities(self.h, self.densities.ctypes.data))
@@ -179,6 +183,15 @@ class DeviceSolver:
         if self._agents is not None and self.agents_count > 0:
             check("bfvm_load_internalized",
                   self.lib.bfvm_load_internalized(self.h, self._agents["internalized_substrates"].ctypes.data))
+
+    def bulk_store_rates(self, supply, uptake, target):
+        """Bulk supply/uptake fields ([V][S] each, the densities' layout): the values of the reference's
+        supply_rate_func, uptake_rate_func and supply_target_densities_func."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (supply, uptake, target)]
+        check("bfvm_bulk_store_rates", self.lib.bfvm_bulk_store_rates(self.h, *[a.ctypes.data for a in arrs]))
+
+    def bulk(self):
+        check("bfvm_bulk_solve", self.lib.bfvm_bulk_solve(self.h))
 
     def diffusion(self):
         check("bfvm_diffusion_solve", self.lib.bfvm_diffusion_solve(self.h))

@@ -430,6 +430,54 @@ static void builder_and_timestep()
 		EXPECT_FLOAT_EQ(m.gradients[i] + 1.0f, 1.0f);
 }
 
+// bulk_solver through the facade, the callbacks sampled at initialize: per voxel, with S = x+1, U = y, T = z+2
+// per substrate index s (scaled by s+1), n solves from D0 = 1 against the closed form of the recurrence
+// D' = (D + dt*S*T) / (1 + dt*(U+S)) evaluated in double precision.
+static void bulk_D3(bool from_team)
+{
+	auto m = default_microenv(cartesian_mesh(3, { 0, 0, 0 }, { 100, 80, 60 }, { 20, 20, 20 }));
+	m.supply_rate_func = [](microenvironment& mm, point_t<index_t, 3> v, real_t* out) {
+		for (index_t s = 0; s < mm.substrates_count; s++)
+			out[s] = (real_t)(v[0] + 1) * (s + 1);
+	};
+	m.uptake_rate_func = [](microenvironment& mm, point_t<index_t, 3> v, real_t* out) {
+		for (index_t s = 0; s < mm.substrates_count; s++)
+			out[s] = (real_t)v[1] * (s + 1);
+	};
+	m.supply_target_densities_func = [](microenvironment& mm, point_t<index_t, 3> v, real_t* out) {
+		for (index_t s = 0; s < mm.substrates_count; s++)
+			out[s] = (real_t)(v[2] + 2);
+	};
+	auto& s = the_solver();
+	s.initialize(m);
+	const int n = 3;
+	auto run = [&] {
+		s.store_data_to_solver(m);
+		for (int i = 0; i < n; i++)
+			s.bulk.solve(m);
+		s.load_data_from_solver(m);
+	};
+	if (from_team)
+	{
+#pragma omp parallel num_threads(4)
+		run();
+	}
+	else
+		run();
+	const double dt = m.diffusion_time_step;
+	for (index_t z = 0; z < m.mesh.grid_shape[2]; z++)
+		for (index_t y = 0; y < m.mesh.grid_shape[1]; y++)
+			for (index_t x = 0; x < m.mesh.grid_shape[0]; x++)
+				for (index_t sb = 0; sb < 2; sb++)
+				{
+					const double S = (x + 1.0) * (sb + 1), U = (double)y * (sb + 1), T = z + 2.0;
+					double d = 1;
+					for (int i = 0; i < n; i++)
+						d = (d + dt * S * T) / (1 + dt * (U + S));
+					EXPECT_FLOAT_EQ(dens(m, x, y, z, sb), d);
+				}
+}
+
 struct test_case
 {
 	const char* name;
@@ -463,6 +511,8 @@ int main(int argc, char** argv)
 		{ "gradient_D3", gradient_D3 },
 		{ "dirichlet_boundaries_D3", boundaries_D3 },
 		{ "builder_and_timestep", builder_and_timestep },
+		{ "bulk_D3", [] { bulk_D3(false); } },
+		{ "bulk_D3_from_omp_team", [] { bulk_D3(true); } },
 	};
 	for (index_t dims : { 1, 2, 3 })
 		for (bool ci : { true, false })

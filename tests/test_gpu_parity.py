@@ -425,3 +425,29 @@ def test_gpu_sorted_reduction_agents_outside_and_reupload(pkg, oracle_mod):
     g.cell(True)
     g.load()
     assert_close(g.densities, o.densities, what="re-uploaded")
+
+
+@pytest.mark.parametrize("dims,shape,S", [(3, (19, 11, 7), 3), (2, (33, 17, 1), 4), (1, (50, 1, 1), 2)])
+def test_gpu_bulk_bitwise(dims, shape, S, pkg, oracle_mod):
+    """bulk_solver (host/bulk_solver.cpp:23-35) with per-voxel supply/uptake/target fields: bit-identical to
+    the float oracle over several solves."""
+    p = random_problem(dims, shape, S, seed=83, faces=False, interior=False)
+    o, g = build_pair(pkg, oracle_mod, p)
+    rng = np.random.Generator(np.random.PCG64(89))
+    n = o.densities.size
+    sup, upt, tgt = (rng.uniform(0, 20, size=n).astype(np.float32) for _ in range(3))
+    g.bulk_store_rates(sup, upt, tgt)
+    g.store()
+    for _ in range(3):
+        o.bulk(sup, upt, tgt)
+        g.bulk()
+    g.load()
+    assert_bitwise(g.densities, o.densities, "bulk")
+
+
+def test_gpu_bulk_requires_rates(pkg, oracle_mod):
+    p = random_problem(3, (5, 4, 3), 2, seed=97, faces=False, interior=False)
+    _, g = build_pair(pkg, oracle_mod, p)
+    with pytest.raises(pkg.BfvmError) as e:
+        g.bulk()
+    assert e.value.code == -2

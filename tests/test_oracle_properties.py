@@ -165,3 +165,25 @@ def test_single_agent_secretion_closed_form(internalized, oracle_mod):
             if internalized:
                 np.testing.assert_allclose(o.internalized[k], I0[k] - vv * ((-den * D + num) / (1 + den) + fac),
                                            rtol=1e-10, atol=1e-10)
+
+
+def test_bulk_closed_forms(oracle_mod):
+    """host/bulk_solver.cpp:23-35, D' = (D + dt*S*T) / (1 + dt*(U+S)), pinned by its closed forms in FP64:
+    without uptake D_n - T = (D_0 - T) / (1 + dt*S)^n; without supply D_n = D_0 / (1 + dt*U)^n; and one
+    general step satisfies D'(1 + dt*(U+S)) - dt*S*T = D."""
+    o, _, _, _ = make(oracle_mod, 3, (7, 5, 6), 3, np.float64, seed=17)
+    rng = np.random.Generator(np.random.PCG64(5))
+    n = o.densities.size
+    S, U, T = (rng.uniform(0, 50, size=n) for _ in range(3))
+    dt = 0.01
+    d0 = o.densities.copy()
+    o.bulk(S, np.zeros(n), T)
+    o.bulk(S, np.zeros(n), T)
+    np.testing.assert_allclose(o.densities - T, (d0 - T) / (1 + dt * S) ** 2, rtol=1e-12, atol=1e-12)
+    d0 = o.densities.copy()
+    for _ in range(3):
+        o.bulk(np.zeros(n), U, T)
+    np.testing.assert_allclose(o.densities, d0 / (1 + dt * U) ** 3, rtol=1e-12)
+    d0 = o.densities.copy()
+    o.bulk(S, U, T)
+    np.testing.assert_allclose(o.densities * (1 + dt * (U + S)) - dt * S * T, d0, rtol=1e-12, atol=1e-12)

@@ -506,7 +506,50 @@ __device__ __forceinline__ typename vec_of<V>::type faces_slow(typename vec_of<V
 // (a block of 16 float4 values plus coefficients does not fit the 128 of a 1024-thread workgroup).
 constexpr int max_group_threads(int V) { return V == 4 ? 512 : 1024; }
 
-// Groups g_first, g_first + g_step, ... < g_end of one partitioned sweep, by one workgroup.  The caller
+// Unit-diagonal Thomas solve of one line's 2P-row reduced system in LDS: rows r[q * rs] (in place), tables
+// L*inv, inv and c' at red + field * rstride + q * S.  The LDS reads of step q+1 are issued before the
+// arithmetic of step q, so each step of the recurrence is one multiply-add on the critical path.
+template <int V>
+__device__ __forceinline__ void reduced_solve(const float* red, size_t rstride, int S, typename vec_of<V>::type* r,
+											  int rs, int nq)
+{
+#pragma clang fp contract(fast)
+	using vec = typename vec_of<V>::type;
+	const float* L = red;
+	const float* inv = red + rstride;
+	const float* cp = red + 2 * rstride;
+	vec Rp = r[0];
+	vec Rn = r[rs], Ln = vload<V>(L + S), In = vload<V>(inv + S);
+#pragma unroll 1
+	for (int q = 1; q < nq; q++)
+	{
+		const vec Rq = Rn, Lq = Ln, Iq = In;
+		if (q + 1 < nq)
+		{
+			Rn = r[(q + 1) * rs];
+			Ln = vload<V>(L + (q + 1) * S);
+			In = vload<V>(inv + (q + 1) * S);
+		}
+		Rp = Rq * Iq - Lq * Rp;
+		r[q * rs] = Rp;
+	}
+	vec u = Rp;
+	vec Rb = r[(nq - 2) * rs], Cb = vload<V>(cp + (nq - 2) * S);
+#pragma unroll 1
+	for (int q = nq - 2; q >= 0; q--)
+	{
+		const vec Rq = Rb, Cq = Cb;
+		if (q > 0)
+		{
+			Rb = r[(q - 1) * rs];
+			Cb = vload<V>(cp + (q - 1) * S);
+		}
+		u = Rq - Cq * u;
+		r[q * rs] = u;
+	}
+}
+
+// Groups g_first, g_first + g_step, ... < g_end of one partitioned sweep, by one workgroup.// Groups g_first, g_first + g_step, ... < g_end of one partitioned sweep, by one workgroup.  The caller
 // synchronises the workgroup before reusing `lds`.
 template <int M, int AXIS, int V, int K>
 __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int64_t g_first, int64_t g_end,
@@ -607,42 +650,14 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 		__syncthreads();
 		// reduced system of the line, one thread per line; the LDS reads of step q+1 are issued before the
 		// arithmetic of step q, and the table holds L*inv, so each step of the recurrence is one multiply-add
-		if (active && k == 0)
+		// with V substrates per line, threads k = 0..V-1 of the line take one substrate each (when the line has
+		// at least V blocks), so one instruction advances every component
+		if (active)
 		{
-			const float* L = lred + 0 * rstride + s0;
-			const float* inv = lred + 1 * rstride + s0;
-			const float* cp = lred + 2 * rstride + s0;
-			vec* r = rv + line_local;
-			const int nq = 2 * P;
-			vec Rp = r[0];
-			vec Rn = r[lpg], Ln = vload<V>(L + S), In = vload<V>(inv + S);
-#pragma unroll 1
-			for (int q = 1; q < nq; q++)
-			{
-				const vec Rq = Rn, Lq = Ln, Iq = In;
-				if (q + 1 < nq)
-				{
-					Rn = r[(q + 1) * lpg];
-					Ln = vload<V>(L + (q + 1) * S);
-					In = vload<V>(inv + (q + 1) * S);
-				}
-				Rp = Rq * Iq - Lq * Rp;
-				r[q * lpg] = Rp;
-			}
-			vec u = Rp;
-			vec Rb = r[(nq - 2) * lpg], Cb = vload<V>(cp + (nq - 2) * S);
-#pragma unroll 1
-			for (int q = nq - 2; q >= 0; q--)
-			{
-				const vec Rq = Rb, Cq = Cb;
-				if (q > 0)
-				{
-					Rb = r[(q - 1) * lpg];
-					Cb = vload<V>(cp + (q - 1) * S);
-				}
-				u = Rq - Cq * u;
-				r[q * lpg] = u;
-			}
+			const int c0 = P >= V ? k : 0;
+			const int c1 = P >= V ? k + 1 : (k == 0 ? V : 0);
+			for (int c = c0; c < c1 && c < V; c++)
+				reduced_solve<1>(lred + s0 + c, rstride, S, (float*)rv + line_local * V + c, lpg * V, 2 * P);
 		}
 		__syncthreads();
 		if (active)
@@ -1137,20 +1152,21 @@ static void free_partition_tables(axis_tables& t)
 }
 
 // Lines handled by one workgroup and its thread count for a partition of P blocks.  `request` is the
-// number of lines per workgroup for y/z (16, 32 or 64 consecutive columns; 0 = 32) and the number of
-// x rows per workgroup for x (0 = as many rows as fit in 256 threads).
+// number of lines per workgroup for y/z and the number of x rows per workgroup for x; 0 picks the shapes
+// measured fastest on 400^3 x 8 (x: as many rows as fit in 200 threads; y: 8 lines of 4-substrate vectors,
+// z: 16; one substrate per thread: 32).
 static void group_shape(int axis, int SG, int P, int request, int max_threads, int& lpg, int& threads)
 {
 	if (axis == 0)
 	{
 		const int per_row = SG * P; // SG = substrate groups of V per x row
-		const int rows = request > 0 ? request : std::max(1, 256 / per_row);
+		const int rows = request > 0 ? request : std::max(1, 200 / per_row);
 		lpg = rows * SG;
 		threads = ((rows * per_row + 63) / 64) * 64;
 	}
 	else
 	{
-		lpg = request > 0 ? request : (max_threads >= 1024 ? 32 : 16);
+		lpg = request > 0 ? request : (max_threads >= 1024 ? 32 : axis == 1 ? 8 : 16);
 		if (lpg * P > max_threads)
 			lpg = std::max(8, max_threads / P / 8 * 8);
 		threads = ((lpg * P + 63) / 64) * 64;

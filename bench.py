@@ -280,8 +280,8 @@ def main():
     ap.add_argument("--algorithm", choices=["partitioned", "exact", "staged"], default="partitioned")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--kernel-iters", type=int, default=20)
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline stepping")
-    ap.add_argument("--cpu-max-steps", type=int, default=10)
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline stepping")
+    ap.add_argument("--cpu-max-steps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--shape", type=int, nargs=3, default=None, help="override the mesh (smaller runs)")
     ap.add_argument("--agents", type=int, default=None)

@@ -234,7 +234,6 @@ template <int V, int K, int TY>
 __device__ __forceinline__ void gradient_tile(const gradient_args& a, int tile)
 {
 	using vec = typename vec_of<V>::type;
-	// z chunk outermost: the tiles one XCD runs together are neighbouring rows of the same z range
 	// z chunk innermost: the tiles one XCD runs together are consecutive z ranges of one row band, so the
 	// planes where one chunk ends and the next begins are read from its L2 (measured 2.5 % faster in FP32
 	// and 7 % in BF16 than keeping neighbouring row bands together)

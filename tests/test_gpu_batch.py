@@ -138,3 +138,20 @@ def test_batch_benchmark_entry_points(pkg):
                  "gradient_phase", "timestep"):
         assert batch.benchmark(what, 2) > 0, what
     batch.close()
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_batch_16bit_storage(storage, pkg, oracle_mod):
+    """Batches keep 16-bit densities too (one rounding counter for all instances): every instance stays within
+    the storage form's error of the FP32 oracle stepping it alone (relative L2 per instance: FP16 1e-3,
+    BF16 1e-2, the bounds of the single-instance precision tests)."""
+    from parabiofvm_amd.configs import make_batch_config
+
+    cfgs = make_batch_config(seed=13, count=5, edge=(8, 36), substrates=(1, 8), agent_fraction=0.02)
+    batch, oracles, solvers = run_batch(pkg, oracle_mod, cfgs, 10, storage=storage)
+    bound = 1e-3 if storage == "f16" else 1e-2
+    for k, (cfg, o, g) in enumerate(zip(cfgs, oracles, solvers)):
+        d = g.densities.astype(np.float64) - o.densities
+        rel = np.linalg.norm(d) / max(np.linalg.norm(o.densities.astype(np.float64)), 1e-30)
+        assert rel <= bound, f"instance {k} grid {cfg['grid']} S {cfg['S']}: rel L2 {rel:.2e}"
+    batch.close()

@@ -451,3 +451,31 @@ def test_gpu_bulk_requires_rates(pkg, oracle_mod):
     with pytest.raises(pkg.BfvmError) as e:
         g.bulk()
     assert e.value.code == -2
+
+
+@pytest.mark.parametrize("reduction", ["atomic", "sorted"])
+def test_gpu_stress_shape_timestep(reduction, pkg, oracle_mod):
+    """A reduced stress workload: 16 substrates (4-substrate vectors with 4 substrate groups per voxel), heavy
+    same-voxel contention (64 Gaussian clusters), Dirichlet faces and interior voxels, 6 graph-replayed steps
+    against the oracle."""
+    from parabiofvm_amd.configs import apply_config, make_config
+
+    cfg = make_config(4, seed=17, shape=(48, 40, 36), agents=60000)
+    args = (cfg["dims"], cfg["mins"], cfg["maxs"], cfg["voxel_shape"], cfg["S"], cfg["dt"], cfg["D"], cfg["decay"],
+            np.zeros(cfg["S"], dtype=np.float32))
+    o = oracle_mod.Oracle(*args, dtype=np.float32)
+    g = pkg.DeviceSolver(*args)
+    for be in (o, g):
+        apply_config(cfg, be)
+        be.initialize()
+    g.set_secretion_reduction(reduction)
+    g.store()
+    steps = 6
+    for i in range(steps):
+        o.timestep(i % 10 == 0, True)
+        g.timestep(i % 10 == 0, True)
+    g.load()
+    assert_close(g.densities, o.densities, what=f"densities ({reduction})")
+    dmax = max(1.0, float(np.abs(o.densities).max()), float(np.abs(cfg["init"]).max()))
+    assert_internalized(g.internalized, o.internalized, cfg["agents"], cfg["S"], cfg["dt"], steps, dmax,
+                        what="internalized")

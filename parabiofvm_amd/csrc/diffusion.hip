@@ -67,10 +67,18 @@ __global__ void exact_coefficients_kernel(float* __restrict__ b, float* __restri
 // The reduced system of the line has unknowns u_{2k} = first, u_{2k+1} = last of block k, with rows
 //   u_{2k}:   alpha'_0 u_{2k-1} + u_{2k} + gamma'_0 u_{2k+1}
 //   u_{2k+1}: alpha_{m-1} u_{2k} + u_{2k+1} + gamma_{m-1} u_{2k+2}
-// and is solved by the Thomas recurrence with unit diagonal; its coefficients L_q, inv_q, c'_q are stored.
+// i.e. unit diagonal, sub-diagonal L_q and super-diagonal U_q.  It is solved from both ends at once (twisted
+// factorization): rows q <= h = P-1 are eliminated downwards, rows q > h upwards, and the two halves meet in
+// the 2x2 system of rows h, h+1:
+//   top    q <= h: inv_q  = 1 / (1 - L_q c'_{q-1}),  c'_q  = U_q inv_q;   stored f0 = L_q inv_q,  f1 = inv_q,  f2 = c'_q
+//   bottom q >  h: jinv_q = 1 / (1 - U_q a''_{q+1}), a''_q = L_q jinv_q;  stored f0 = U_q jinv_q, f1 = jinv_q, f2 = a''_q
+//   meeting factor mu = 1 / (1 - c'_h a''_{h+1})
 // Table layout: coef[f][type][j][s], f = 0 r, 1 gamma, 2 alpha' (rho in the j=0 slot), 3 gamma';
-// red[f][q][s], f = 0 L, 1 inv, 2 c'.
+// red: 16-byte records {f0, f1, f2, 0} at [q][s], then mu[s].
 constexpr int max_types = 4;
+
+// floats of the reduced-system table of a line of P blocks
+__host__ __device__ constexpr int reduced_table_floats(int P, int S) { return 4 * 2 * P * S + S; }
 
 __global__ void partition_coefficients_kernel(float* __restrict__ coef, float* __restrict__ red,
 											  float* __restrict__ A_out, const float* __restrict__ D,
@@ -164,14 +172,18 @@ __global__ void partition_coefficients_kernel(float* __restrict__ coef, float* _
 		ga_last[t] = gamma[m - 1];
 	}
 
-	// reduced system, Thomas with unit diagonal
-	const size_t rstride = (size_t)2 * P * S;
-	double cprev = 0;
-	for (int q = 0; q < 2 * P; q++)
-	{
+	// reduced system, twisted factorization with unit diagonal
+	const int N = 2 * P, h = P - 1;
+	auto record = [&](int q, double f0, double f1, double f2) {
+		float* rec = red + ((size_t)q * S + s) * 4;
+		rec[0] = (float)f0;
+		rec[1] = (float)f1;
+		rec[2] = (float)f2;
+		rec[3] = 0.f;
+	};
+	auto off_diagonals = [&](int q, double& L, double& U) {
 		const int k = q / 2;
 		const int t = (P == 1) ? 0 : (k == 0 ? 0 : (k == P - 1 ? 3 : (k < rem ? 1 : 2)));
-		double L, U;
 		if ((q & 1) == 0)
 		{
 			L = (k > 0) ? ap0[t] : 0.0;
@@ -182,22 +194,28 @@ __global__ void partition_coefficients_kernel(float* __restrict__ coef, float* _
 			L = al_last[t];
 			U = (k < P - 1) ? ga_last[t] : 0.0;
 		}
-		double inv, cp;
-		if (q == 0)
-		{
-			inv = 1.0;
-			cp = U;
-		}
-		else
-		{
-			inv = 1.0 / (1.0 - L * cprev);
-			cp = U * inv;
-		}
-		red[0 * rstride + (size_t)q * S + s] = (float)(L * inv);
-		red[1 * rstride + (size_t)q * S + s] = (float)inv;
-		red[2 * rstride + (size_t)q * S + s] = (float)cp;
+	};
+	double cprev = 0; // c'_{q-1}
+	for (int q = 0; q <= h; q++)
+	{
+		double L, U;
+		off_diagonals(q, L, U);
+		const double inv = q == 0 ? 1.0 : 1.0 / (1.0 - L * cprev);
+		const double cp = U * inv;
+		record(q, L * inv, inv, cp);
 		cprev = cp;
 	}
+	double anext = 0; // a''_{q+1}
+	for (int q = N - 1; q > h; q--)
+	{
+		double L, U;
+		off_diagonals(q, L, U);
+		const double jinv = q == N - 1 ? 1.0 : 1.0 / (1.0 - U * anext);
+		const double a2 = L * jinv;
+		record(q, U * jinv, jinv, a2);
+		anext = a2;
+	}
+	red[(size_t)N * S * 4 + s] = (float)(1.0 / (1.0 - cprev * anext));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -506,50 +524,75 @@ __device__ __forceinline__ typename vec_of<V>::type faces_slow(typename vec_of<V
 // (a block of 16 float4 values plus coefficients does not fit the 128 of a 1024-thread workgroup).
 constexpr int max_group_threads(int V) { return V == 4 ? 512 : 1024; }
 
-// Unit-diagonal Thomas solve of one line's 2P-row reduced system in LDS: rows r[q * rs] (in place), tables
-// L*inv, inv and c' at red + field * rstride + q * S.  The LDS reads of step q+1 are issued before the
-// arithmetic of step q, so each step of the recurrence is one multiply-add on the critical path.
-template <int V>
-__device__ __forceinline__ void reduced_solve(const float* red, size_t rstride, int S, typename vec_of<V>::type* r,
-											  int rs, int nq)
+// Twisted Thomas solve of one line's 2P-row reduced system, run by two adjacent lanes of a wave: half 0
+// eliminates rows 1..h downwards, half 1 rows 2P-2..h+1 upwards (h = P-1), the lanes swap their last values,
+// both resolve rows h and h+1, and each substitutes back towards its own end.  Rows r[q * rs] in LDS, in
+// place; table records {f0, f1, f2} at tab[q * S] and mu (both offset to the line's substrate).  Each step is
+// one multiply-add on the critical path, and the LDS reads of step i+1 are issued before the arithmetic of
+// step i.  The chain is P steps each way instead of 2P for a one-ended solve.
+__device__ __forceinline__ void twisted_solve(const float4* tab, float mu, int S, float* r, int rs, int P, int half)
 {
 #pragma clang fp contract(fast)
-	using vec = typename vec_of<V>::type;
-	const float* L = red;
-	const float* inv = red + rstride;
-	const float* cp = red + 2 * rstride;
-	vec Rp = r[0];
-	vec Rn = r[rs], Ln = vload<V>(L + S), In = vload<V>(inv + S);
-#pragma unroll 1
-	for (int q = 1; q < nq; q++)
+	const int h = P - 1;
+	const int dr = half ? -rs : rs;
+	const int dt = half ? -S : S;
+	float* rp = r + (half ? (2 * P - 1) * rs : 0);
+	const float4* tp = tab + (half ? (2 * P - 1) * S : 0);
+	float y = *rp; // f1 is 1 in both end rows
+	float Rn = 0.f;
+	float2 Fn = make_float2(0.f, 0.f);
+	if (h > 0)
 	{
-		const vec Rq = Rn, Lq = Ln, Iq = In;
-		if (q + 1 < nq)
-		{
-			Rn = r[(q + 1) * rs];
-			Ln = vload<V>(L + (q + 1) * S);
-			In = vload<V>(inv + (q + 1) * S);
-		}
-		Rp = Rq * Iq - Lq * Rp;
-		r[q * rs] = Rp;
+		Rn = rp[dr];
+		Fn = *(const float2*)(tp + dt);
 	}
-	vec u = Rp;
-	vec Rb = r[(nq - 2) * rs], Cb = vload<V>(cp + (nq - 2) * S);
 #pragma unroll 1
-	for (int q = nq - 2; q >= 0; q--)
+	for (int i = 1; i <= h; i++)
 	{
-		const vec Rq = Rb, Cq = Cb;
-		if (q > 0)
+		rp += dr;
+		tp += dt;
+		const float Rq = Rn;
+		const float2 Fq = Fn;
+		if (i < h)
 		{
-			Rb = r[(q - 1) * rs];
-			Cb = vload<V>(cp + (q - 1) * S);
+			Rn = rp[dr];
+			Fn = *(const float2*)(tp + dt);
 		}
-		u = Rq - Cq * u;
-		r[q * rs] = u;
+		y = Rq * Fq.y - Fq.x * y;
+		*rp = y;
+	}
+	// rp, tp are at row h (top lane) or h+1 (bottom lane), y is that row's eliminated value; both lanes solve
+	//   u_h = (top - c'_h bottom) mu,  u_{h+1} = bottom - a''_{h+1} u_h
+	const float other = __shfl_xor(y, 1);
+	const float top = half ? other : y;
+	const float bottom = half ? y : other;
+	const float uh = (top - tab[h * S].z * bottom) * mu;
+	y = half ? bottom - tp->z * uh : uh;
+	*rp = y;
+	// back substitution towards the lane's end: u_q = r_q - f2_q u_{q -+ 1}
+	float Cn = 0.f;
+	if (h > 0)
+	{
+		Rn = rp[-dr];
+		Cn = tp[-dt].z;
+	}
+#pragma unroll 1
+	for (int i = 1; i <= h; i++)
+	{
+		rp -= dr;
+		tp -= dt;
+		const float Rq = Rn, Cq = Cn;
+		if (i < h)
+		{
+			Rn = rp[-dr];
+			Cn = tp[-dt].z;
+		}
+		y = Rq - Cq * y;
+		*rp = y;
 	}
 }
 
-// Groups g_first, g_first + g_step, ... < g_end of one partitioned sweep, by one workgroup.// Groups g_first, g_first + g_step, ... < g_end of one partitioned sweep, by one workgroup.  The caller
+// Groups g_first, g_first + g_step, ... < g_end of one partitioned sweep, by one workgroup.  The caller
 // synchronises the workgroup before reusing `lds`.
 template <int M, int AXIS, int V, int K>
 __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int64_t g_first, int64_t g_end,
@@ -563,7 +606,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 	const int P = a.P;
 	const int lpg = a.lines_per_group;
 	const int ncoef = 4 * a.ntypes * a.mb * S;
-	const int nred = 3 * 2 * P * S;
+	const int nred = reduced_table_floats(P, S);
 	float* lcoef = lds;
 	float* lred = lds + ncoef;
 	vec* rv = (vec*)(lred + nred); // [2P][lpg]
@@ -578,7 +621,6 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 	const int64_t stride = line_stride<AXIS>(g);
 	const int grid[3] = { g.X, g.Y, g.Z };
 	const size_t fstride = (size_t)a.ntypes * a.mb * S;
-	const size_t rstride = (size_t)2 * P * S;
 	const uint32_t key = K == store_f32 ? 0u : rounding_key(g.rounding, 1 + AXIS);
 
 	int start = 0, mk = 0, type = 0;
@@ -648,16 +690,19 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 					rv[(2 * k + 1) * lpg + line_local] = v[j];
 		}
 		__syncthreads();
-		// reduced system of the line, one thread per line; the LDS reads of step q+1 are issued before the
-		// arithmetic of step q, and the table holds L*inv, so each step of the recurrence is one multiply-add
-		// with V substrates per line, threads k = 0..V-1 of the line take one substrate each (when the line has
-		// at least V blocks), so one instruction advances every component
-		if (active)
+		// reduced systems: one pair of adjacent threads per (line, substrate), solving from both ends.  On every
+		// axis the first substrate of line l is (l V) mod S (the lines of a row, column or plane position run
+		// over the substrate groups), and the group's share of it is uniform across the workgroup.
+		const int group_s = (int)((group * lpg * V) % S);
+		for (int pr = threadIdx.x >> 1; threadIdx.x < (blockDim.x & ~1u) && pr < lpg * V; pr += blockDim.x >> 1)
 		{
-			const int c0 = P >= V ? k : 0;
-			const int c1 = P >= V ? k + 1 : (k == 0 ? V : 0);
-			for (int c = c0; c < c1 && c < V; c++)
-				reduced_solve<1>(lred + s0 + c, rstride, S, (float*)rv + line_local * V + c, lpg * V, 2 * P);
+			const int ll = pr / V, c = pr - ll * V;
+			if (group * lpg + ll < a.lines)
+			{
+				const int sc = (group_s + ll * V) % S + c;
+				twisted_solve((const float4*)lred + sc, lred[8 * P * S + sc], S, (float*)rv + ll * V + c, lpg * V, P,
+							  threadIdx.x & 1);
+			}
 		}
 		__syncthreads();
 		if (active)
@@ -891,7 +936,7 @@ __device__ __forceinline__ void sweep_tiles(const tile_args& t, float* lds, int6
 	const int P = a.P;
 	const int lpp = t.pass_lines;
 	const int ncoef = 4 * a.ntypes * a.mb * S;
-	const int nred = 3 * 2 * P * S;
+	const int nred = reduced_table_floats(P, S);
 	float* lcoef = lds;
 	float* lred = lds + ncoef;
 	float* rv = lred + nred; // [2P][lpp]
@@ -917,7 +962,6 @@ __device__ __forceinline__ void sweep_tiles(const tile_args& t, float* lds, int6
 		type = (P == 1) ? 0 : (k == 0 ? 0 : (k == P - 1 ? 3 : (k < a.rem ? 1 : 2)));
 	}
 	const size_t fstride = (size_t)a.ntypes * a.mb * S;
-	const size_t rstride = (size_t)2 * P * S;
 	const uint32_t key = K == store_f32 ? 0u : rounding_key(g.rounding, 1 + AXIS);
 	const int grid[3] = { g.X, g.Y, g.Z };
 	const int64_t XS = (int64_t)g.X * S;
@@ -1012,41 +1056,14 @@ __device__ __forceinline__ void sweep_tiles(const tile_args& t, float* lds, int6
 						rv[(2 * k + 1) * lpp + line_local] = x[j];
 			}
 			__syncthreads();
-			if (active && k == 0)
+			// reduced systems: one pair of adjacent threads per line of the pass, solving from both ends
+			for (int pr = threadIdx.x >> 1; threadIdx.x < (blockDim.x & ~1u) && pr < lpp; pr += blockDim.x >> 1)
 			{
-				const float* L = lred + 0 * rstride + s;
-				const float* inv = lred + 1 * rstride + s;
-				const float* cp = lred + 2 * rstride + s;
-				float* r = rv + line_local;
-				const int nq = 2 * P;
-				float Rp = r[0];
-				float Rn = r[lpp], Ln = L[S], In = inv[S];
-#pragma unroll 1
-				for (int q = 1; q < nq; q++)
+				const int pl = l0 + pr;
+				if (pl < v.lines)
 				{
-					const float Rq = Rn, Lq = Ln, Iq = In;
-					if (q + 1 < nq)
-					{
-						Rn = r[(q + 1) * lpp];
-						Ln = L[(q + 1) * S];
-						In = inv[(q + 1) * S];
-					}
-					Rp = Rq * Iq - Lq * Rp;
-					r[q * lpp] = Rp;
-				}
-				float u = Rp;
-				float Rb = r[(nq - 2) * lpp], Cb = cp[(nq - 2) * S];
-#pragma unroll 1
-				for (int q = nq - 2; q >= 0; q--)
-				{
-					const float Rq = Rb, Cq = Cb;
-					if (q > 0)
-					{
-						Rb = r[(q - 1) * lpp];
-						Cb = cp[(q - 1) * S];
-					}
-					u = Rq - Cq * u;
-					r[q * lpp] = u;
+					const int ps = AXIS == 0 ? pl % S : (int)(((AXIS == 1 ? v.q0 : v.first) + pl) % S);
+					twisted_solve((const float4*)lred + ps, lred[8 * P * S + ps], S, rv + pr, lpp, P, threadIdx.x & 1);
 				}
 			}
 			__syncthreads();
@@ -1176,7 +1193,7 @@ static void group_shape(int axis, int SG, int P, int request, int max_threads, i
 static size_t partition_lds_bytes(const partition& p, int S, int lpg)
 {
 	return sizeof(float) *
-		   ((size_t)4 * p.ntypes * p.mb * S + (size_t)6 * p.P * S + (size_t)2 * p.P * lpg * p.vec);
+		   ((size_t)4 * p.ntypes * p.mb * S + (size_t)reduced_table_floats(p.P, S) + (size_t)2 * p.P * lpg * p.vec);
 }
 
 partition choose_partition(const bfvm_ctx* ctx, int axis, int chunk, int lines_request, int max_vec)
@@ -1230,7 +1247,7 @@ partition choose_partition(const bfvm_ctx* ctx, int axis, int chunk, int lines_r
 void alloc_partition_tables(const partition& p, int S, float*& coef, float*& red, float*& A)
 {
 	BFVM_HIP(hipMalloc(&coef, sizeof(float) * 4 * p.ntypes * p.mb * S));
-	BFVM_HIP(hipMalloc(&red, sizeof(float) * 3 * 2 * p.P * S));
+	BFVM_HIP(hipMalloc(&red, sizeof(float) * reduced_table_floats(p.P, S)));
 	BFVM_HIP(hipMalloc(&A, sizeof(float) * S));
 }
 
@@ -1370,7 +1387,7 @@ static tile_args make_tile_args(const bfvm_ctx* ctx, int axis, const sweep_geome
 	const int64_t XS = (int64_t)g.X * g.S;
 	const int n = p.n;
 	const int max_lines = std::max(1, 1024 / p.P);
-	const size_t tables = sizeof(float) * ((size_t)4 * p.ntypes * p.mb * g.S + (size_t)6 * p.P * g.S +
+	const size_t tables = sizeof(float) * ((size_t)4 * p.ntypes * p.mb * g.S + (size_t)reduced_table_floats(p.P, g.S) +
 										   (size_t)2 * p.P * max_lines + 4);
 	if (tables + 16 * n * e > staged_lds_limit)
 		throw std::runtime_error("staged sweep: coefficient tables leave no room for a tile");
@@ -1402,7 +1419,8 @@ static tile_args make_tile_args(const bfvm_ctx* ctx, int axis, const sweep_geome
 static size_t tile_lds_bytes(const tile_args& t, int S, int storage)
 {
 	const int P = t.a.P;
-	const size_t tables = (size_t)4 * t.a.ntypes * t.a.mb * S + (size_t)6 * P * S + (size_t)2 * P * t.pass_lines;
+	const size_t tables =
+		(size_t)4 * t.a.ntypes * t.a.mb * S + (size_t)reduced_table_floats(P, S) + (size_t)2 * P * t.pass_lines;
 	return ((tables + 3) & ~(size_t)3) * sizeof(float) + (size_t)t.tile_elems * storage_bytes(storage);
 }
 

@@ -197,11 +197,11 @@ __global__ __launch_bounds__(256) void gradient_kernel(float* __restrict__ grad,
 }
 
 // 3-D gradient, one thread per (TY rows of y, q) column group (q = x*S + s, V consecutive substrates of one
-// voxel per thread) marching along z with the z-1, z, z+1 densities of its rows in registers; x neighbours
-// and the y rows just outside the thread's rows are read from the current plane.  Tiles of 256 threads are
-// numbered so that consecutive tiles run on the same XCD and share its L2: the dispatcher hands workgroup b
-// to XCD b % 8, so workgroup b takes tile (b % 8) * per_xcd + b / 8.  Index arithmetic is 32-bit per plane;
-// plane offsets are 64-bit.
+// voxel per thread) marching along z with the z-1, z, z+1 densities of its rows in registers.  The x
+// neighbours come from the lanes holding voxel x +- 1 (cross-lane reads), the y rows just outside the
+// thread's rows are read from the current plane.  Tiles of 256 threads are numbered so that consecutive tiles
+// run on the same XCD and share its L2: the dispatcher hands workgroup b to XCD b % 8, so workgroup b takes
+// tile (b % 8) * per_xcd + b / 8.  Index arithmetic is 32-bit per plane; plane offsets are 64-bit.
 constexpr int grad_tile = 256;
 constexpr int num_xcds = 8;
 
@@ -230,6 +230,19 @@ constexpr int gradient_rows()
 	return (K == store_f32 && V == 4) ? 8 : 4;
 }
 
+constexpr int wave_lanes = 64;
+
+// v as held by lane `src` of this wave (ds_bpermute; src is taken modulo the wave size)
+template <int V>
+__device__ __forceinline__ typename vec_of<V>::type shfl_vec(typename vec_of<V>::type v, int src)
+{
+	typename vec_of<V>::type r;
+#pragma unroll
+	for (int c = 0; c < V; c++)
+		set_comp(r, c, __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(comp(v, c)))));
+	return r;
+}
+
 template <int V, int K, int TY>
 __device__ __forceinline__ void gradient_tile(const gradient_args& a, int tile)
 {
@@ -253,6 +266,8 @@ __device__ __forceinline__ void gradient_tile(const gradient_args& a, int tile)
 	float* g1 = g0 + a.VS;
 	float* g2 = g1 + a.VS;
 	const bool xlo = q < S, xhi = q >= XS - S;
+	const int lane = threadIdx.x & (wave_lanes - 1);
+	const int SG = S / V; // lanes per voxel
 	const float bx = a.bx, cx = a.cx, by = a.by, cy = a.cy, bz = a.bz, cz = a.cz;
 	const void* dens = a.dens;
 
@@ -290,8 +305,18 @@ __device__ __forceinline__ void gradient_tile(const gradient_args& a, int tile)
 				break;
 			const int y = y0 + r;
 			const int64_t orow = o + (int64_t)r * XS;
-			const vec xp = xhi ? vec(0.f) : ld(orow + S);
-			const vec xm = xlo ? vec(0.f) : ld(orow - S);
+			// x neighbours: the same substrates of voxel x +- 1 sit SG lanes away in this wave's registers;
+			// only the lanes at the wave's edges load theirs
+			vec xp = shfl_vec<V>(c[r], lane + SG);
+			vec xm = shfl_vec<V>(c[r], lane - SG);
+			if (xhi)
+				xp = vec(0.f);
+			else if (lane + SG >= wave_lanes)
+				xp = ld(orow + S);
+			if (xlo)
+				xm = vec(0.f);
+			else if (lane < SG)
+				xm = ld(orow - S);
 			const vec ym = r == 0 ? below : c[r - 1 < 0 ? 0 : r - 1];
 			const vec yp = r == ny - 1 ? above : c[r + 1 < TY ? r + 1 : TY - 1];
 			vec gx, gy, gz;

@@ -26,6 +26,21 @@ struct vec_of<1>
 	using type = float;
 };
 
+// Accesses through the global address space.  Device buffers all live in global memory, but pointers read
+// from memory (the per-instance arguments of the batched kernels) are generic to the compiler, which then
+// emits flat accesses (counted against the LDS wait counter too); loads and stores of HBM data go through
+// these helpers so they are global accesses wherever the pointer came from.
+template <typename T>
+__device__ __forceinline__ T gload(const T* p)
+{
+	return *(const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, T v)
+{
+	*(__attribute__((address_space(1))) T*)p = v;
+}
+
 template <int V>
 __device__ __forceinline__ typename vec_of<V>::type vload(const float* p)
 {
@@ -91,7 +106,7 @@ template <int K, int V>
 __device__ __forceinline__ typename vec_of<V>::type dload(const void* base, int64_t idx)
 {
 	using st = typename svec_of<K, V>::type;
-	const st v = *(const st*)((const typename store_of<K>::elem*)base + idx);
+	const st v = gload((const st*)((const typename store_of<K>::elem*)base + idx));
 	if constexpr (K == store_f32)
 		return v;
 	else if constexpr (V == 1)
@@ -106,11 +121,11 @@ __device__ __forceinline__ void dstore(void* base, int64_t idx, typename vec_of<
 	using st = typename svec_of<K, V>::type;
 	st* p = (st*)((typename store_of<K>::elem*)base + idx);
 	if constexpr (K == store_f32)
-		*p = x;
+		gstore(p, x);
 	else if constexpr (V == 1)
-		*p = (typename store_of<K>::elem)x;
+		gstore(p, (typename store_of<K>::elem)x);
 	else
-		*p = __builtin_convertvector(x, st);
+		gstore(p, __builtin_convertvector(x, st));
 }
 
 // 32-bit integer hash (xorshift-multiply avalanche, "lowbias32" constants of C. Wellons' hash-prospector).
@@ -138,6 +153,33 @@ __device__ __forceinline__ uint32_t rounding_key(const unsigned* counter, uint32
 // conversion is exact.  Random bits: one hash of the element index with `key` per vector, split into 16-bit
 // halves, and a multiplicative-congruential step (an odd multiplier, a bijection of the 32-bit state) for
 // every further pair of components.  FP32 stores unchanged.
+//
+// gfx950 does this conversion in one instruction (v_cvt_sr_bf16_f32 / v_cvt_sr_f16_f32): it adds the top
+// 16 (BF16) or 13 (FP16) bits of its random operand below the kept mantissa and truncates, which is the
+// scheme above with the random bits taken from the top of the word (measured bit-exact against the
+// software form on 2^20 random values).  Component c uses h for even c and h << 16 for odd c.
+
+// Two values converted with stochastic rounding into one pair of 16-bit elements: `a` with random word h,
+// `b` with h << 16.
+template <int K>
+__device__ __forceinline__ typename store_of<K>::elem __attribute__((ext_vector_type(2)))
+sr_pair(float a, float b, uint32_t h)
+{
+	using pair = typename store_of<K>::elem __attribute__((ext_vector_type(2)));
+	pair o = {};
+	if constexpr (K == store_bf16)
+	{
+		o = __builtin_amdgcn_cvt_sr_bf16_f32(o, a, h, false);
+		o = __builtin_amdgcn_cvt_sr_bf16_f32(o, b, h << 16, true);
+	}
+	else
+	{
+		o = __builtin_amdgcn_cvt_sr_f16_f32(o, a, h, false);
+		o = __builtin_amdgcn_cvt_sr_f16_f32(o, b, h << 16, true);
+	}
+	return o;
+}
+
 template <int K, int V>
 __device__ __forceinline__ void dstore_sr(void* base, int64_t idx, typename vec_of<V>::type x, uint32_t key)
 {
@@ -145,19 +187,25 @@ __device__ __forceinline__ void dstore_sr(void* base, int64_t idx, typename vec_
 		dstore<K, V>(base, idx, x);
 	else
 	{
-		constexpr uint32_t drop = K == store_bf16 ? 0xFFFFu : 0x1FFFu;
+		using elem = typename store_of<K>::elem;
+		elem* p = (elem*)base + idx;
 		uint32_t h = mix32((uint32_t)idx ^ ((uint32_t)((uint64_t)idx >> 32) * 0x9E3779B9u) ^ key);
-		typename vec_of<V>::type y;
-#pragma unroll
-		for (int c = 0; c < V; c++)
+		if constexpr (V == 1)
+			gstore(p, sr_pair<K>(x, 0.f, h)[0]);
+		else
 		{
-			if (c > 0 && (c & 1) == 0)
-				h = h * 0x2C1B3C6Du + 0x297A2D39u;
-			const uint32_t r = (c & 1) ? (h >> 16) : h;
-			const uint32_t b = (__float_as_uint(comp(x, c)) + (r & drop)) & ~drop;
-			set_comp(y, c, __uint_as_float(b));
+			typename svec_of<K, V>::type y;
+#pragma unroll
+			for (int c = 0; c < V; c += 2)
+			{
+				if (c > 0)
+					h = h * 0x2C1B3C6Du + 0x297A2D39u;
+				const auto o = sr_pair<K>(comp(x, c), comp(x, c + 1), h);
+				y[c] = o[0];
+				y[c + 1] = o[1];
+			}
+			gstore((typename svec_of<K, V>::type*)p, y);
 		}
-		dstore<K, V>(base, idx, y);
 	}
 }
 
@@ -342,12 +390,16 @@ struct bfvm_batch
 	unsigned* rounding = nullptr; // step counter of the stochastic rounding, shared by all instances
 	struct sweep_plan
 	{
-		void* inst = nullptr; // device part_args per instance
-		bfvm::batch_item* items = nullptr;
-		int nitems = 0;
-		size_t lds = 0;
-		int threads = 0;
-		int blocks = 0;
+		// instances grouped by the vector width of their threads (4, 1), one launch per group
+		struct part
+		{
+			void* inst = nullptr; // device part_args per instance
+			bfvm::batch_item* items = nullptr;
+			int nitems = 0;
+			size_t lds = 0;
+			int threads = 0;
+			int blocks = 0;
+		} parts[2];
 		std::vector<float*> tables; // coefficient tables owned by the batch
 		std::vector<int> single;	// instances whose mesh has length 1 on this axis (per-instance launch)
 		bool any = false;

@@ -79,6 +79,14 @@ constexpr int max_types = 4;
 
 // floats of the reduced-system table of a line of P blocks
 __host__ __device__ constexpr int reduced_table_floats(int P, int S) { return 4 * 2 * P * S + S; }
+// LDS of a partitioned sweep workgroup, in floats: the coefficient table [4][ntypes][mb][Sp], the reduced-system
+// table, A, 1/scale and scale [Sp] each, then (from the returned offset, 16-byte aligned) the reduced rows.  Sp is
+// S, or S + V - 1 when substrate vectors wrap (rows extended periodically so V consecutive entries from any
+// substrate are the wrapped substrates).
+__host__ __device__ constexpr int part_rows_offset(int ntypes, int mb, int P, int S, int Sp)
+{
+	return (4 * ntypes * mb * Sp + reduced_table_floats(P, S) + 3 * Sp + 3) / 4 * 4;
+}
 
 __global__ void partition_coefficients_kernel(float* __restrict__ coef, float* __restrict__ red,
 											  float* __restrict__ A_out, const float* __restrict__ D,
@@ -495,21 +503,53 @@ __device__ __forceinline__ void thread_slot(int t, int SG, int P, int lpg, int& 
 	}
 }
 
+// Substrate s and voxel coordinates cc of component c of a thread's substrate vector (first substrate s0,
+// line coordinates `coord`).  Without wrapping the V substrates belong to one voxel.  With wrapping (y/z
+// sweeps whose substrate count V does not divide) the vector holds V consecutive flattened plane positions,
+// so component c is substrate (s0 + c) mod S of a voxel (s0 + c) / S further along x (and, along z, possibly
+// in the next y row).
+template <int AXIS, bool WRAP>
+__device__ __forceinline__ void comp_coords(const sweep_geometry& g, const int* coord, int s0, int c, int& s, int* cc)
+{
+	cc[0] = coord[0];
+	cc[1] = coord[1];
+	cc[2] = coord[2];
+	if constexpr (!WRAP)
+		s = s0 + c;
+	else
+	{
+		static_assert(AXIS != 0, "x lines hold whole voxels");
+		const int t = s0 + c;
+		s = t % g.S;
+		const int adv = t / g.S;
+		if constexpr (AXIS == 1)
+			cc[0] += adv;
+		else
+		{
+			const int xx = coord[0] + adv;
+			cc[0] = xx % g.X;
+			cc[1] += xx / g.X;
+		}
+	}
+}
+
 // Dirichlet faces for the elements a thread stores: the common case (an interior line away from its ends)
 // stores the solved value; lines on a face of another axis, and the two ends of the line, resolve every
 // face in the order x-, x+, y-, y+, z-, z+ (host/dirichlet_solver.cpp:91-110).
-template <int AXIS, int V>
+template <int AXIS, int V, bool WRAP>
 __device__ __forceinline__ typename vec_of<V>::type faces_slow(typename vec_of<V>::type x, const sweep_geometry& g,
 															  int s0, const int* coord, int i)
 {
 	const int grid[3] = { g.X, g.Y, g.Z };
 	for (int c = 0; c < V; c++)
 	{
-		const int s = s0 + c;
+		int s = s0 + c, cc[3];
+		if constexpr (WRAP)
+			comp_coords<AXIS, WRAP>(g, coord, s0, c, s, cc);
 		float val = comp(x, c);
 		for (int d = 0; d < g.dims; d++)
 		{
-			const int cd = d == AXIS ? i : coord[d];
+			const int cd = d == AXIS ? i : (WRAP ? cc[d] : coord[d]);
 			if (cd == 0 && g.fcond[(2 * d) * g.S + s])
 				val = g.fval[(2 * d) * g.S + s];
 			if (cd == grid[d] - 1 && g.fcond[(2 * d + 1) * g.S + s])
@@ -609,12 +649,22 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 	const int nred = reduced_table_floats(P, S);
 	float* lcoef = lds;
 	float* lred = lds + ncoef;
-	vec* rv = (vec*)(lred + nred); // [2P][lpg]
+	// per-substrate A, 1/scale and scale, so the group loop reads no global memory besides the densities
+	float* lA = lred + nred;
+	float* linv = lA + S;
+	float* lscale = linv + S;
+	vec* rv = (vec*)(lds + part_rows_offset(a.ntypes, a.mb, P, S, S)); // [2P][lpg]
 
 	for (int i = threadIdx.x; i < ncoef; i += blockDim.x)
 		lcoef[i] = a.coef[i];
 	for (int i = threadIdx.x; i < nred; i += blockDim.x)
 		lred[i] = a.red[i];
+	for (int i = threadIdx.x; i < S; i += blockDim.x)
+	{
+		lA[i] = a.A[i];
+		linv[i] = K == store_f32 ? 1.f : g.inv[i];
+		lscale[i] = K == store_f32 ? 1.f : g.scale[i];
+	}
 
 	int line_local, k;
 	thread_slot<AXIS>(threadIdx.x, S / V, P, lpg, line_local, k);
@@ -656,14 +706,14 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 					v[j] = dload<K, V>(g.data, off0 + j * stride);
 			if constexpr (K != store_f32)
 			{
-				const vec inv = vload<V>(g.inv + s0);
+				const vec inv = vload<V>(linv + s0);
 #pragma unroll
 				for (int j = 0; j < M; j++)
 					v[j] = v[j] * inv;
 			}
 			// forward elimination inside the block: v_j = r_j (v_j - A v_{j-1}), written as r_j v_j - (r_j A) v_{j-1}
 			// so that the recurrence carries one multiply-add
-			const vec As = vload<V>(a.A + s0);
+			const vec As = vload<V>(lA + s0);
 #pragma unroll
 			for (int j = 0; j < M; j++)
 			{
@@ -712,7 +762,13 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 			// faces touching this line: another axis's face through the whole line, or this axis's faces at
 			// the line ends
 			bool line_face = false, end_min = false, end_max = false;
-			if (g.fcond)
+			// only lines on a face of another axis and the end blocks of a line can meet a face: the
+			// conditions are read for those alone
+			bool near_face = start == 0 || start + mk == a.n;
+			for (int d = 0; d < g.dims; d++)
+				if (d != AXIS)
+					near_face |= coord[d] == 0 || coord[d] == grid[d] - 1;
+			if (g.fcond && near_face)
 			{
 				for (int c = 0; c < V; c++)
 					for (int d = 0; d < g.dims; d++)
@@ -728,6 +784,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 							line_face |= (cmin && coord[d] == 0) || (cmax && coord[d] == grid[d] - 1);
 					}
 			}
+			const vec sc = vload<V>(lscale + s0);
 #pragma unroll
 			for (int j = 0; j < M; j++)
 			{
@@ -742,9 +799,235 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 						x = v[j] - vload<V>(ca + j * S) * u0 - vload<V>(cgp + j * S) * u1;
 					const int i = start + j;
 					if (line_face || (i == 0 && end_min) || (i == a.n - 1 && end_max))
-						x = faces_slow<AXIS, V>(x, g, s0, coord, i);
+						x = faces_slow<AXIS, V, false>(x, g, s0, coord, i);
 					if constexpr (K != store_f32)
-						x = x * vload<V>(g.scale + s0);
+						x = x * sc;
+					dstore_sr<K, V>(g.data, off0 + j * stride, x, key);
+				}
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// sweep_groups for y/z lines whose threads hold V consecutive flattened plane positions that may wrap over
+// the substrates (V need not divide S): per-substrate tables in LDS extended periodically to S + V - 1 entries
+// and read component by component, per-component substrate and voxel coordinates for the faces.
+template <int M, int AXIS, int V, int K, bool WRAP = true>
+__device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* lds, int64_t g_first, int64_t g_end,
+											 int64_t g_step)
+{
+	// this path is held to a tolerance, not to the reference's operation order: multiply-adds may fuse
+#pragma clang fp contract(fast)
+	using vec = typename vec_of<V>::type;
+	const sweep_geometry& g = a.g;
+	const int S = g.S;
+	const int P = a.P;
+	const int lpg = a.lines_per_group;
+	const int Sp = WRAP ? S + V - 1 : S; // LDS row length of the per-substrate tables
+	const int ncoef = 4 * a.ntypes * a.mb * Sp;
+	const int nred = reduced_table_floats(P, S);
+	float* lcoef = lds;
+	float* lred = lds + ncoef;
+	// per-substrate A, 1/scale and scale, so the group loop reads no global memory besides the densities
+	float* lA = lred + nred;
+	float* linv = lA + Sp;
+	float* lscale = linv + Sp;
+	vec* rv = (vec*)(lds + part_rows_offset(a.ntypes, a.mb, P, S, Sp)); // [2P][lpg]
+
+	if constexpr (WRAP)
+	{
+		for (int i = threadIdx.x; i < ncoef; i += blockDim.x)
+		{
+			const int row = i / Sp;
+			lcoef[i] = a.coef[row * S + (i - row * Sp) % S];
+		}
+	}
+	else
+	{
+		for (int i = threadIdx.x; i < ncoef; i += blockDim.x)
+			lcoef[i] = a.coef[i];
+	}
+	for (int i = threadIdx.x; i < nred; i += blockDim.x)
+		lred[i] = a.red[i];
+	for (int i = threadIdx.x; i < Sp; i += blockDim.x)
+	{
+		const int si = i % S;
+		lA[i] = a.A[si];
+		linv[i] = K == store_f32 ? 1.f : g.inv[si];
+		lscale[i] = K == store_f32 ? 1.f : g.scale[si];
+	}
+	// V table entries from substrate s0 on: one vector read, or component by component where wrapping makes the
+	// start unaligned
+	auto cld = [](const float* p) -> vec {
+		if constexpr (WRAP)
+		{
+			vec r;
+#pragma unroll
+			for (int c = 0; c < V; c++)
+				set_comp(r, c, p[c]);
+			return r;
+		}
+		else
+			return vload<V>(p);
+	};
+
+	int line_local, k;
+	thread_slot<AXIS>(threadIdx.x, AXIS == 0 ? S / V : 1, P, lpg, line_local, k);
+	const int64_t stride = line_stride<AXIS>(g);
+	const int grid[3] = { g.X, g.Y, g.Z };
+	const size_t fstride = (size_t)a.ntypes * a.mb * Sp;
+	const uint32_t key = K == store_f32 ? 0u : rounding_key(g.rounding, 1 + AXIS);
+
+	int start = 0, mk = 0, type = 0;
+	if (k >= 0)
+	{
+		start = k * a.base + min(k, a.rem);
+		mk = a.base + (k < a.rem ? 1 : 0);
+		type = (P == 1) ? 0 : (k == 0 ? 0 : (k == P - 1 ? 3 : (k < a.rem ? 1 : 2)));
+	}
+	__syncthreads();
+
+	for (int64_t group = g_first; group < g_end; group += g_step)
+	{
+		const int64_t line = group * lpg + line_local;
+		const bool active = k >= 0 && line < a.lines;
+		int64_t base = 0;
+		int s0 = 0;
+		int coord[3] = { 0, 0, 0 };
+		if (active)
+			vline_coords<AXIS, V>(g, line, base, s0, coord);
+		const int64_t off0 = base + (int64_t)start * stride; // element offset of the block's first value
+		const float* cr = lcoef + 0 * fstride + (size_t)type * a.mb * Sp + s0;
+		const float* cg = lcoef + 1 * fstride + (size_t)type * a.mb * Sp + s0;
+		const float* ca = lcoef + 2 * fstride + (size_t)type * a.mb * Sp + s0;
+		const float* cgp = lcoef + 3 * fstride + (size_t)type * a.mb * Sp + s0;
+
+		vec v[M];
+		if (active)
+		{
+#pragma unroll
+			for (int j = 0; j < M; j++)
+				if (j < mk)
+					v[j] = dload<K, V>(g.data, off0 + j * stride);
+			if constexpr (K != store_f32)
+			{
+				const vec inv = cld(linv + s0);
+#pragma unroll
+				for (int j = 0; j < M; j++)
+					v[j] = v[j] * inv;
+			}
+			// forward elimination inside the block: v_j = r_j (v_j - A v_{j-1}), written as r_j v_j - (r_j A) v_{j-1}
+			// so that the recurrence carries one multiply-add
+			const vec As = cld(lA + s0);
+#pragma unroll
+			for (int j = 0; j < M; j++)
+			{
+				if (j < mk)
+				{
+					const vec r = cld(cr + j * Sp);
+					if (j < 2)
+						v[j] = r * v[j];
+					else
+						v[j] = r * v[j] - (r * As) * v[j - 1];
+				}
+			}
+			// backward elimination towards the block's last unknown
+#pragma unroll
+			for (int j = M - 3; j >= 1; j--)
+				if (j <= mk - 3)
+					v[j] = v[j] - cld(cg + j * Sp) * v[j + 1];
+			if (mk >= 3)
+				v[0] = cld(ca) * (v[0] - cld(cg) * v[1]);
+			rv[(2 * k) * lpg + line_local] = v[0];
+#pragma unroll
+			for (int j = 1; j < M; j++)
+				if (j == mk - 1)
+					rv[(2 * k + 1) * lpg + line_local] = v[j];
+		}
+		__syncthreads();
+		// reduced systems: one pair of adjacent threads per (line, substrate), solving from both ends.  On every
+		// axis the first substrate of line l is (l V) mod S (the lines of a row, column or plane position run
+		// over the substrate groups), and the group's share of it is uniform across the workgroup.
+		const int group_s = (int)((group * lpg * V) % S);
+		for (int pr = threadIdx.x >> 1; threadIdx.x < (blockDim.x & ~1u) && pr < lpg * V; pr += blockDim.x >> 1)
+		{
+			const int ll = pr / V, c = pr - ll * V;
+			if (group * lpg + ll < a.lines)
+			{
+				const int sc = WRAP ? (group_s + ll * V + c) % S : (group_s + ll * V) % S + c;
+				twisted_solve((const float4*)lred + sc, lred[8 * P * S + sc], S, (float*)rv + ll * V + c, lpg * V, P,
+							  threadIdx.x & 1);
+			}
+		}
+		__syncthreads();
+		if (active)
+		{
+			const vec u0 = rv[(2 * k) * lpg + line_local];
+			const vec u1 = rv[(2 * k + 1) * lpg + line_local];
+			// faces touching this line: another axis's face through the whole line, or this axis's faces at
+			// the line ends
+			bool line_face = false, end_min = false, end_max = false;
+			// only lines on a face of another axis and the end blocks of a line can meet a face: the
+			// conditions are read for those alone
+			bool near_face = start == 0 || start + mk == a.n;
+			if constexpr (WRAP)
+			{
+				for (int c = 0; c < V; c++)
+				{
+					int sc, cc[3];
+					comp_coords<AXIS, WRAP>(g, coord, s0, c, sc, cc);
+					for (int d = 0; d < g.dims; d++)
+						if (d != AXIS)
+							near_face |= cc[d] == 0 || cc[d] == grid[d] - 1;
+				}
+			}
+			else
+			{
+				for (int d = 0; d < g.dims; d++)
+					if (d != AXIS)
+						near_face |= coord[d] == 0 || coord[d] == grid[d] - 1;
+			}
+			if (g.fcond && near_face)
+			{
+				for (int c = 0; c < V; c++)
+				{
+					int sc = s0 + c, cc[3];
+					if constexpr (WRAP)
+						comp_coords<AXIS, WRAP>(g, coord, s0, c, sc, cc);
+					for (int d = 0; d < g.dims; d++)
+					{
+						const bool cmin = g.fcond[(2 * d) * S + sc] != 0;
+						const bool cmax = g.fcond[(2 * d + 1) * S + sc] != 0;
+						const int cd = WRAP ? cc[d] : coord[d];
+						if (d == AXIS)
+						{
+							end_min |= cmin;
+							end_max |= cmax;
+						}
+						else
+							line_face |= (cmin && cd == 0) || (cmax && cd == grid[d] - 1);
+					}
+				}
+			}
+			const vec sc = cld(lscale + s0);
+#pragma unroll
+			for (int j = 0; j < M; j++)
+			{
+				if (j < mk)
+				{
+					vec x;
+					if (j == 0)
+						x = u0;
+					else if (j == mk - 1)
+						x = u1;
+					else
+						x = v[j] - cld(ca + j * Sp) * u0 - cld(cgp + j * Sp) * u1;
+					const int i = start + j;
+					if (line_face || (i == 0 && end_min) || (i == a.n - 1 && end_max))
+						x = faces_slow<AXIS, V, WRAP>(x, g, s0, coord, i);
+					if constexpr (K != store_f32)
+						x = x * sc;
 					dstore_sr<K, V>(g.data, off0 + j * stride, x, key);
 				}
 			}
@@ -760,21 +1043,22 @@ __global__ __launch_bounds__(max_group_threads(V)) void sweep_partitioned_kernel
 	sweep_groups<M, AXIS, V, K>(a, lds, blockIdx.x, a.groups, gridDim.x);
 }
 
-// Many microenvironments in one launch: workgroups walk a work list of (instance, group range) items;
-// every instance runs one substrate per thread with 16-element register blocks (any substrate count), and
-// its workgroup shape fits the launch's block size.  (Splitting the batch by substrates-per-thread into
-// separate launches measured no faster.)
-template <int AXIS, int K>
-__global__ __launch_bounds__(batch_max_threads) void sweep_batched_kernel(const part_args* __restrict__ inst,
-																		 const batch_item* __restrict__ items,
-																		 int nitems)
+// Many microenvironments in one launch: workgroups walk a work list of (instance, group range) items.  The
+// instances of one launch have V substrates (x) or flattened plane positions (y, z) per thread, with 16-element
+// register blocks; on y and z the V positions may cross voxels (substrate vectors wrap).
+template <int AXIS, int K, int V>
+__global__ __launch_bounds__(V == 1 ? batch_max_threads : 512) void sweep_batched_kernel(
+	const part_args* __restrict__ inst, const batch_item* __restrict__ items, int nitems)
 {
 	extern __shared__ float lds[];
 	for (int it = blockIdx.x; it < nitems; it += gridDim.x)
 	{
 		const batch_item w = items[it];
 		const part_args a = inst[w.instance];
-		sweep_groups<batch_chunk, AXIS, 1, K>(a, lds, w.begin, w.end, 1);
+		if constexpr (V > 1 && AXIS != 0)
+			sweep_groups_wrapped<batch_chunk, AXIS, V, K>(a, lds, w.begin, w.end, 1);
+		else
+			sweep_groups<batch_chunk, AXIS, V, K>(a, lds, w.begin, w.end, 1);
 	}
 }
 
@@ -1190,10 +1474,10 @@ static void group_shape(int axis, int SG, int P, int request, int max_threads, i
 	}
 }
 
-static size_t partition_lds_bytes(const partition& p, int S, int lpg)
+static size_t partition_lds_bytes(const partition& p, int S, int lpg, bool wrap = false)
 {
-	return sizeof(float) *
-		   ((size_t)4 * p.ntypes * p.mb * S + (size_t)reduced_table_floats(p.P, S) + (size_t)2 * p.P * lpg * p.vec);
+	const int Sp = wrap ? S + p.vec - 1 : S;
+	return sizeof(float) * ((size_t)part_rows_offset(p.ntypes, p.mb, p.P, S, Sp) + (size_t)2 * p.P * lpg * p.vec);
 }
 
 partition choose_partition(const bfvm_ctx* ctx, int axis, int chunk, int lines_request, int max_vec)
@@ -1515,24 +1799,40 @@ void free_batch_sweep(bfvm_batch* b, int axis)
 		if (t)
 			BFVM_HIP(hipFree(t));
 	ax.tables.clear();
-	if (ax.inst)
-		BFVM_HIP(hipFree(ax.inst));
-	if (ax.items)
-		BFVM_HIP(hipFree(ax.items));
+	for (auto& part : ax.parts)
+	{
+		if (part.inst)
+			BFVM_HIP(hipFree(part.inst));
+		if (part.items)
+			BFVM_HIP(hipFree(part.items));
+	}
 	ax = bfvm_batch::sweep_plan {};
 }
 
-// Per instance: the partition with 16-element register blocks, one substrate per thread and as many lines
-// per group as fit batch_group_threads; its own coefficient tables; work items of about 64K elements.
+// Per instance: the partition with 16-element register blocks; its own coefficient tables; work items of about
+// 64K elements.  Each thread holds V = 4 or 1 consecutive elements of a row: on x, V substrates of a voxel
+// (V divides S); on y and z, V flattened plane positions (V divides the row or plane length, so every vector
+// is aligned), which wrap over the substrates when V does not divide S.  4-element vectors move 16 (FP32) or
+// 8 bytes per lane, against 4 or 2 for one element.  (Two-element vectors made the compiler move the register
+// blocks through scratch memory, so they are not used.)
+static int batch_vector_width(const bfvm_ctx* ctx, int axis)
+{
+	const int S = ctx->mesh.S;
+	const int64_t XS = (int64_t)ctx->mesh.grid[0] * S;
+	const int64_t unit = axis == 0 ? S : (axis == 1 ? XS : (int64_t)ctx->mesh.grid[1] * XS);
+	return unit % 4 == 0 ? 4 : 1;
+}
+
 void prepare_batch_sweep(bfvm_batch* b, int axis)
 {
 	auto& ax = b->axes[axis];
-	std::vector<part_args> args(b->ctx.size());
-	std::vector<batch_item> items;
+	std::vector<part_args> args[2];
+	std::vector<batch_item> items[2];
+	for (auto& a : args)
+		a.assign(b->ctx.size(), part_args {});
 	for (size_t i = 0; i < b->ctx.size(); i++)
 	{
 		bfvm_ctx* ctx = b->ctx[i];
-		args[i] = part_args {};
 		if (axis >= ctx->mesh.dims)
 			continue;
 		const int n = ctx->mesh.grid[axis];
@@ -1542,13 +1842,21 @@ void prepare_batch_sweep(bfvm_batch* b, int axis)
 			continue;
 		}
 		const int S = ctx->mesh.S;
-		const int P = (n + batch_chunk - 1) / batch_chunk;
-		const int request = axis == 0 ? std::max(1, batch_group_threads / (S * P))
-									  : std::max(8, batch_group_threads / P / 8 * 8);
-		const partition p = choose_partition(ctx, axis, batch_chunk, request, 1);
-		if (p.chunk != batch_chunk || p.vec != 1 || p.threads > batch_max_threads)
+		const int V = batch_vector_width(ctx, axis);
+		const int w = V == 4 ? 0 : 1;
+		const int max_threads = V == 1 ? batch_max_threads : 512;
+		partition p = choose_partition(ctx, axis, batch_chunk, 1, 1);
+		if (p.chunk != batch_chunk)
 			throw std::runtime_error("batched sweep: instance " + std::to_string(i) +
-									 " does not fit one workgroup of 16-element blocks (axis too long)");
+									 " does not fit 16-element register blocks (axis too long)");
+		p.vec = V;
+		const int SG = axis == 0 ? S / V : 1;
+		const int request = axis == 0 ? std::max(1, batch_group_threads / (SG * p.P))
+									  : std::max(8, batch_group_threads / p.P / 8 * 8);
+		group_shape(axis, SG, p.P, request, max_threads, p.lpg, p.threads);
+		if (p.threads > max_threads)
+			throw std::runtime_error("batched sweep: instance " + std::to_string(i) +
+									 " does not fit one workgroup (axis too long)");
 		float *coef = nullptr, *red = nullptr, *A = nullptr;
 		alloc_partition_tables(p, S, coef, red, A);
 		ax.tables.push_back(coef);
@@ -1557,31 +1865,44 @@ void prepare_batch_sweep(bfvm_batch* b, int axis)
 		launch_partition_tables(ctx, axis, p, coef, red, A, b->stream);
 		sweep_geometry g = geometry(ctx);
 		g.rounding = b->rounding;
-		args[i] = make_part_args(g, p, coef, red, A);
-		ax.lds = std::max(ax.lds, partition_lds_bytes(p, S, p.lpg));
-		ax.threads = std::max(ax.threads, p.threads);
-		const int64_t per_group = (int64_t)p.lpg * n;
+		args[w][i] = make_part_args(g, p, coef, red, A);
+		auto& part = ax.parts[w];
+		part.lds = std::max(part.lds, partition_lds_bytes(p, S, p.lpg, V > 1 && axis != 0));
+		part.threads = std::max(part.threads, p.threads);
+		const int64_t per_group = (int64_t)p.lpg * n * V;
 		const int64_t per_item = std::max<int64_t>(1, 65536 / per_group);
-		for (int64_t g0 = 0; g0 < args[i].groups; g0 += per_item)
-			items.push_back(batch_item { (int32_t)i, 0, g0, std::min(args[i].groups, g0 + per_item) });
+		for (int64_t g0 = 0; g0 < args[w][i].groups; g0 += per_item)
+			items[w].push_back(batch_item { (int32_t)i, 0, g0, std::min(args[w][i].groups, g0 + per_item) });
 		ax.any = true;
 	}
-	if (!ax.any)
-		return;
-	BFVM_HIP(hipMalloc(&ax.inst, sizeof(part_args) * args.size()));
-	BFVM_HIP(hipMemcpy(ax.inst, args.data(), sizeof(part_args) * args.size(), hipMemcpyHostToDevice));
-	BFVM_HIP(hipMalloc(&ax.items, sizeof(batch_item) * items.size()));
-	BFVM_HIP(hipMemcpy(ax.items, items.data(), sizeof(batch_item) * items.size(), hipMemcpyHostToDevice));
-	ax.nitems = (int)items.size();
-	ax.blocks = std::min(ax.nitems, 2048);
+	for (int w = 0; w < 2; w++)
+	{
+		auto& part = ax.parts[w];
+		if (items[w].empty())
+			continue;
+		if (part.lds > 64 * 1024)
+			throw std::runtime_error("batched sweep: coefficient tables exceed 64 KB of LDS");
+		BFVM_HIP(hipMalloc(&part.inst, sizeof(part_args) * args[w].size()));
+		BFVM_HIP(hipMemcpy(part.inst, args[w].data(), sizeof(part_args) * args[w].size(), hipMemcpyHostToDevice));
+		BFVM_HIP(hipMalloc(&part.items, sizeof(batch_item) * items[w].size()));
+		BFVM_HIP(hipMemcpy(part.items, items[w].data(), sizeof(batch_item) * items[w].size(), hipMemcpyHostToDevice));
+		part.nitems = (int)items[w].size();
+		part.blocks = std::min(part.nitems, 2048);
+	}
 }
 
 template <int AXIS, int K>
 static void launch_batched_t(bfvm_batch* b)
 {
 	const auto& ax = b->axes[AXIS];
-	hipLaunchKernelGGL((sweep_batched_kernel<AXIS, K>), dim3(ax.blocks), dim3(ax.threads), ax.lds, b->stream,
-					   (const part_args*)ax.inst, ax.items, ax.nitems);
+	const auto& p4 = ax.parts[0];
+	const auto& p1 = ax.parts[1];
+	if (p4.nitems)
+		hipLaunchKernelGGL((sweep_batched_kernel<AXIS, K, 4>), dim3(p4.blocks), dim3(p4.threads), p4.lds, b->stream,
+						   (const part_args*)p4.inst, p4.items, p4.nitems);
+	if (p1.nitems)
+		hipLaunchKernelGGL((sweep_batched_kernel<AXIS, K, 1>), dim3(p1.blocks), dim3(p1.threads), p1.lds, b->stream,
+						   (const part_args*)p1.inst, p1.items, p1.nitems);
 	BFVM_HIP(hipGetLastError());
 }
 

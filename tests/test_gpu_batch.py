@@ -155,3 +155,24 @@ def test_batch_16bit_storage(storage, pkg, oracle_mod):
         rel = np.linalg.norm(d) / max(np.linalg.norm(o.densities.astype(np.float64)), 1e-30)
         assert rel <= bound, f"instance {k} grid {cfg['grid']} S {cfg['S']}: rel L2 {rel:.2e}"
     batch.close()
+
+
+def test_batch_vector_widths_with_dirichlet(pkg, oracle_mod):
+    """Instances whose rows take 4-, 2- and 1-element vectors per thread, with substrate counts those widths do
+    not divide (the vectors of y and z then wrap over the substrates and cross voxels), Dirichlet conditions on
+    every face with some substrates' conditions off, and interior Dirichlet voxels; 6 steps against the oracle."""
+    from parabiofvm_amd.configs import make_config
+
+    rng = np.random.default_rng(17)
+    # (grid, S): x*S % 4 == 0 with odd S (y: 4-wide, wrapping), x*S % 4 == 2 (2-wide), odd x*S (1-wide),
+    # S = 6 (x: 2-wide; y, z: 4-wide wrapping), S = 1 (4 voxels per vector), S = 4 and S = 2 (no wrapping)
+    shapes = [((36, 20, 18), 3), ((10, 9, 21), 5), ((9, 11, 7), 3), ((8, 6, 19), 6), ((16, 5, 6), 1),
+              ((7, 18, 9), 4), ((33, 5, 7), 2)]
+    cfgs = []
+    for k, (grid, S) in enumerate(shapes):
+        c = make_config(1, seed=100 + k, shape=grid, substrates=S, agents=60)
+        c["faces"] = {key: (vals, (rng.random(S) < 0.7).astype(np.uint8)) for key, (vals, _) in c["faces"].items()}
+        cfgs.append(c)
+    batch, oracles, solvers = run_batch(pkg, oracle_mod, cfgs, 6)
+    check_instances(cfgs, oracles, solvers, 6)
+    batch.close()

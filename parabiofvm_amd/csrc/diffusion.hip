@@ -238,8 +238,9 @@ struct face_epilogue
 	bool lo_set = false, hi_set = false, cmin = false, cmax = false;
 	float lo_val = 0, hi_val = 0, vmin = 0, vmax = 0;
 
+	// `scale`: the storage scale of substrate s, for sweeps that solve on stored (scaled) values
 	__device__ void init(const float* __restrict__ fval, const uint8_t* __restrict__ fcond, int S, int s, int axis,
-						 const int* coord, const int* grid, int dims)
+						 const int* coord, const int* grid, int dims, float scale = 1.f)
 	{
 		if (!fval)
 			return;
@@ -249,7 +250,7 @@ struct face_epilogue
 			{
 				const int f = 2 * d + mx;
 				const bool cond = fcond[f * S + s] != 0;
-				const float val = fval[f * S + s];
+				const float val = fval[f * S + s] * scale;
 				if (d == axis)
 				{
 					if (mx == 0)
@@ -536,9 +537,10 @@ __device__ __forceinline__ void comp_coords(const sweep_geometry& g, const int* 
 // Dirichlet faces for the elements a thread stores: the common case (an interior line away from its ends)
 // stores the solved value; lines on a face of another axis, and the two ends of the line, resolve every
 // face in the order x-, x+, y-, y+, z-, z+ (host/dirichlet_solver.cpp:91-110).
-template <int AXIS, int V, bool WRAP>
+template <int AXIS, int V, bool WRAP, int K>
 __device__ __forceinline__ typename vec_of<V>::type faces_slow(typename vec_of<V>::type x, const sweep_geometry& g,
-															  int s0, const int* coord, int i)
+															  int s0, const int* coord, int i,
+															  const typename vec_of<V>::type& scale)
 {
 	const int grid[3] = { g.X, g.Y, g.Z };
 	for (int c = 0; c < V; c++)
@@ -547,14 +549,25 @@ __device__ __forceinline__ typename vec_of<V>::type faces_slow(typename vec_of<V
 		if constexpr (WRAP)
 			comp_coords<AXIS, WRAP>(g, coord, s0, c, s, cc);
 		float val = comp(x, c);
+		bool face = false;
 		for (int d = 0; d < g.dims; d++)
 		{
 			const int cd = d == AXIS ? i : (WRAP ? cc[d] : coord[d]);
 			if (cd == 0 && g.fcond[(2 * d) * g.S + s])
+			{
 				val = g.fval[(2 * d) * g.S + s];
+				face = true;
+			}
 			if (cd == grid[d] - 1 && g.fcond[(2 * d + 1) * g.S + s])
+			{
 				val = g.fval[(2 * d + 1) * g.S + s];
+				face = true;
+			}
 		}
+		// 16-bit forms: a face value takes the storage scale (solved values already carry it)
+		if constexpr (K != store_f32)
+			if (face)
+				val *= comp(scale, c);
 		set_comp(x, c, val);
 	}
 	return x;
@@ -704,13 +717,8 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 			for (int j = 0; j < M; j++)
 				if (j < mk)
 					v[j] = dload<K, V>(g.data, off0 + j * stride);
-			if constexpr (K != store_f32)
-			{
-				const vec inv = vload<V>(linv + s0);
-#pragma unroll
-				for (int j = 0; j < M; j++)
-					v[j] = v[j] * inv;
-			}
+			// 16-bit forms solve on the stored values: the scale is a power of two, so solving on scaled values and
+			// scaling the solution give the same bits, and only the Dirichlet values set below need the scale
 			// forward elimination inside the block: v_j = r_j (v_j - A v_{j-1}), written as r_j v_j - (r_j A) v_{j-1}
 			// so that the recurrence carries one multiply-add
 			const vec As = vload<V>(lA + s0);
@@ -799,9 +807,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 						x = v[j] - vload<V>(ca + j * S) * u0 - vload<V>(cgp + j * S) * u1;
 					const int i = start + j;
 					if (line_face || (i == 0 && end_min) || (i == a.n - 1 && end_max))
-						x = faces_slow<AXIS, V, false>(x, g, s0, coord, i);
-					if constexpr (K != store_f32)
-						x = x * sc;
+						x = faces_slow<AXIS, V, false, K>(x, g, s0, coord, i, sc);
 					dstore_sr<K, V>(g.data, off0 + j * stride, x, key);
 				}
 			}
@@ -910,13 +916,8 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 			for (int j = 0; j < M; j++)
 				if (j < mk)
 					v[j] = dload<K, V>(g.data, off0 + j * stride);
-			if constexpr (K != store_f32)
-			{
-				const vec inv = cld(linv + s0);
-#pragma unroll
-				for (int j = 0; j < M; j++)
-					v[j] = v[j] * inv;
-			}
+			// 16-bit forms solve on the stored values: the scale is a power of two, so solving on scaled values and
+			// scaling the solution give the same bits, and only the Dirichlet values set below need the scale
 			// forward elimination inside the block: v_j = r_j (v_j - A v_{j-1}), written as r_j v_j - (r_j A) v_{j-1}
 			// so that the recurrence carries one multiply-add
 			const vec As = cld(lA + s0);
@@ -1025,9 +1026,7 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 						x = v[j] - cld(ca + j * Sp) * u0 - cld(cgp + j * Sp) * u1;
 					const int i = start + j;
 					if (line_face || (i == 0 && end_min) || (i == a.n - 1 && end_max))
-						x = faces_slow<AXIS, V, WRAP>(x, g, s0, coord, i);
-					if constexpr (K != store_f32)
-						x = x * sc;
+						x = faces_slow<AXIS, V, WRAP, K>(x, g, s0, coord, i, sc);
 					dstore_sr<K, V>(g.data, off0 + j * stride, x, key);
 				}
 			}
@@ -1309,11 +1308,11 @@ __device__ __forceinline__ void sweep_tiles(const tile_args& t, float* lds, int6
 			float x[M];
 			if (active)
 			{
-				const float inv = K == store_f32 ? 1.f : g.inv[s];
+				// 16-bit forms solve on the stored (power-of-two scaled) values, as in sweep_groups
 #pragma unroll
 				for (int j = 0; j < M; j++)
 					if (j < mk)
-						x[j] = (float)at(start + j) * inv;
+						x[j] = (float)at(start + j);
 				const float As = a.A[s];
 #pragma unroll
 				for (int j = 0; j < M; j++)
@@ -1356,8 +1355,7 @@ __device__ __forceinline__ void sweep_tiles(const tile_args& t, float* lds, int6
 				const float u0 = rv[(2 * k) * lpp + line_local];
 				const float u1 = rv[(2 * k + 1) * lpp + line_local];
 				face_epilogue fe;
-				fe.init(g.fval, g.fcond, S, s, AXIS, coord, grid, g.dims);
-				const float sc = K == store_f32 ? 1.f : g.scale[s];
+				fe.init(g.fval, g.fcond, S, s, AXIS, coord, grid, g.dims, K == store_f32 ? 1.f : g.scale[s]);
 #pragma unroll
 				for (int j = 0; j < M; j++)
 				{
@@ -1380,7 +1378,7 @@ __device__ __forceinline__ void sweep_tiles(const tile_args& t, float* lds, int6
 							const int64_t gi = gbase + (int64_t)i * gstep;
 							constexpr uint32_t drop = K == store_bf16 ? 0xFFFFu : 0x1FFFu;
 							const uint32_t rb = mix32((uint32_t)gi ^ ((uint32_t)((uint64_t)gi >> 32) * 0x9E3779B9u) ^ key);
-							const uint32_t b = (__float_as_uint(y * sc) + (rb & drop)) & ~drop;
+							const uint32_t b = (__float_as_uint(y) + (rb & drop)) & ~drop;
 							at(i) = (elem)__uint_as_float(b);
 						}
 					}

@@ -115,6 +115,25 @@ __device__ __forceinline__ typename vec_of<V>::type dload(const void* base, int6
 		return __builtin_convertvector(v, typename vec_of<V>::type);
 }
 
+// The same load in two halves: the raw stored elements, and their conversion to FP32 (for loads issued one
+// loop iteration ahead of their use)
+template <int K, int V>
+__device__ __forceinline__ typename svec_of<K, V>::type draw(const void* base, int64_t idx)
+{
+	using st = typename svec_of<K, V>::type;
+	return gload((const st*)((const typename store_of<K>::elem*)base + idx));
+}
+template <int K, int V>
+__device__ __forceinline__ typename vec_of<V>::type dcvt(typename svec_of<K, V>::type v)
+{
+	if constexpr (K == store_f32)
+		return v;
+	else if constexpr (V == 1)
+		return (float)v;
+	else
+		return __builtin_convertvector(v, typename vec_of<V>::type);
+}
+
 template <int K, int V>
 __device__ __forceinline__ void dstore(void* base, int64_t idx, typename vec_of<V>::type x)
 {

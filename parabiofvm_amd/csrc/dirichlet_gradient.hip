@@ -288,16 +288,49 @@ __device__ __forceinline__ void gradient_tile(const gradient_args& a, int tile)
 		zm[r] = (r < ny && z0 > 0) ? ld((int64_t)(z0 - 1) * plane + (int64_t)r * XS) : vec(0.f);
 		c[r] = r < ny ? ld((int64_t)z0 * plane + (int64_t)r * XS) : vec(0.f);
 	}
+	// 16-bit forms keep one plane step of loads in flight: plane z + 1 and the y neighbours of plane z are
+	// loaded (raw) during the step before, as a z step is otherwise bound by the latency of its own loads
+	constexpr bool prefetch = K != store_f32;
+	using raw = typename svec_of<K, V>::type;
+	raw pzp[prefetch ? TY : 1], pbelow, pabove;
+	auto fetch = [&](int zz) {
+		const int64_t oo = (int64_t)zz * plane;
+#pragma unroll
+		for (int r = 0; r < TY; r++)
+			if (r < ny && zz + 1 < Z)
+				pzp[r] = draw<K, V>(dens, off + oo + plane + (int64_t)r * XS);
+		if (y0 > 0)
+			pbelow = draw<K, V>(dens, off + oo - XS);
+		if (y0 + ny < Y)
+			pabove = draw<K, V>(dens, off + oo + (int64_t)ny * XS);
+	};
+	if constexpr (prefetch)
+		if (z0 < z1)
+			fetch(z0);
 	for (int z = z0; z < z1; z++)
 	{
 		const int64_t o = (int64_t)z * plane;
 		vec zp[TY];
+		vec below, above;
+		if constexpr (prefetch)
+		{
 #pragma unroll
-		for (int r = 0; r < TY; r++)
-			zp[r] = (r < ny && z + 1 < Z) ? ld(o + plane + (int64_t)r * XS) : vec(0.f);
-		// rows y0-1 and y0+ny (the y neighbours outside this thread's rows)
-		const vec below = y0 > 0 ? ld(o - XS) : vec(0.f);
-		const vec above = y0 + ny < Y ? ld(o + (int64_t)ny * XS) : vec(0.f);
+			for (int r = 0; r < TY; r++)
+				zp[r] = (r < ny && z + 1 < Z) ? dcvt<K, V>(pzp[r]) * is : vec(0.f);
+			below = y0 > 0 ? dcvt<K, V>(pbelow) * is : vec(0.f);
+			above = y0 + ny < Y ? dcvt<K, V>(pabove) * is : vec(0.f);
+			if (z + 1 < z1)
+				fetch(z + 1);
+		}
+		else
+		{
+#pragma unroll
+			for (int r = 0; r < TY; r++)
+				zp[r] = (r < ny && z + 1 < Z) ? ld(o + plane + (int64_t)r * XS) : vec(0.f);
+			// rows y0-1 and y0+ny (the y neighbours outside this thread's rows)
+			below = y0 > 0 ? ld(o - XS) : vec(0.f);
+			above = y0 + ny < Y ? ld(o + (int64_t)ny * XS) : vec(0.f);
+		}
 #pragma unroll
 		for (int r = 0; r < TY; r++)
 		{

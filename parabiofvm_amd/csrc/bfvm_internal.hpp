@@ -418,7 +418,6 @@ struct bfvm_batch
 			size_t lds = 0;
 			int threads = 0;
 			int blocks = 0;
-			bool staged = false; // inst holds tile_args of LDS-staged sweeps instead of part_args
 		} parts[2];
 		std::vector<float*> tables; // coefficient tables owned by the batch
 		std::vector<int> single;	// instances whose mesh has length 1 on this axis (per-instance launch)

@@ -1134,7 +1134,7 @@ __global__ __launch_bounds__(V == 1 ? batch_max_threads : 512) void sweep_batche
 		if constexpr (V > 1 && AXIS != 0)
 			sweep_groups_wrapped<batch_chunk, AXIS, V, K>(a, lds, w.begin, w.end, 1);
 		else
-			sweep_groups<batch_chunk, AXIS, V, K, (K != store_f32 && V > 1)>(a, lds, w.begin, w.end, 1);
+			sweep_groups<batch_chunk, AXIS, V, K, (K != store_f32)>(a, lds, w.begin, w.end, 1);
 	}
 }
 
@@ -1475,21 +1475,6 @@ __global__ __launch_bounds__(1024) void sweep_staged_kernel(tile_args t)
 	sweep_tiles<AXIS, K>(t, lds, blockIdx.x, t.tiles, gridDim.x);
 }
 
-// Many microenvironments, LDS-staged: workgroups walk a work list of (instance, tile range) items.
-template <int AXIS, int K>
-__global__ __launch_bounds__(1024) void sweep_batched_staged_kernel(const tile_args* __restrict__ inst,
-																	const batch_item* __restrict__ items, int nitems)
-{
-	extern __shared__ float lds[];
-	for (int it = blockIdx.x; it < nitems; it += gridDim.x)
-	{
-		const batch_item w = items[it];
-		const tile_args t = inst[w.instance];
-		sweep_tiles<AXIS, K>(t, lds, w.begin, w.end, 1);
-		__syncthreads();
-	}
-}
-
 // ------------------------------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------------------------------
@@ -1753,13 +1738,9 @@ static void launch_exact_axis(bfvm_ctx* ctx, const sweep_geometry& g, const axis
 // stays within 64 KB (tiles of at most 48 KB); one pass solves up to 1024 / P lines.
 static constexpr size_t staged_lds_limit = 64 * 1024;
 static constexpr size_t staged_tile_bytes = 48 * 1024;
-// batched staged sweeps: smaller tiles and passes, so several workgroups share a CU and overlap their phases
-static constexpr size_t batch_tile_bytes = 32 * 1024;
-static constexpr int batch_tile_threads = 512;
 
 static tile_args make_tile_args(const bfvm_ctx* ctx, int axis, const sweep_geometry& g, const partition& p,
-								const float* coef, const float* red, const float* A,
-								size_t tile_bytes = staged_tile_bytes, int max_threads = 1024)
+								const float* coef, const float* red, const float* A)
 {
 	tile_args t;
 	t.a = make_part_args(g, p, coef, red, A);
@@ -1767,12 +1748,12 @@ static tile_args make_tile_args(const bfvm_ctx* ctx, int axis, const sweep_geome
 	const int VE = (int)(16 / e);
 	const int64_t XS = (int64_t)g.X * g.S;
 	const int n = p.n;
-	const int max_lines = std::max(1, max_threads / p.P);
+	const int max_lines = std::max(1, 1024 / p.P);
 	const size_t tables = sizeof(float) * ((size_t)4 * p.ntypes * p.mb * g.S + (size_t)reduced_table_floats(p.P, g.S) +
 										   (size_t)2 * p.P * max_lines + 4);
 	if (tables + 16 * n * e > staged_lds_limit)
 		throw std::runtime_error("staged sweep: coefficient tables leave no room for a tile");
-	const size_t budget = std::min(tile_bytes, staged_lds_limit - tables);
+	const size_t budget = std::min(staged_tile_bytes, staged_lds_limit - tables);
 	if (axis == 0)
 	{
 		const int64_t rows = (int64_t)g.Y * g.Z;
@@ -1924,14 +1905,9 @@ void prepare_batch_sweep(bfvm_batch* b, int axis)
 {
 	auto& ax = b->axes[axis];
 	std::vector<part_args> args[2];
-	std::vector<tile_args> tiles(b->ctx.size(), tile_args {});
 	std::vector<batch_item> items[2];
 	for (auto& a : args)
 		a.assign(b->ctx.size(), part_args {});
-	// one element per thread: 16-bit lanes move bytes at about half the rate of 32-bit ones, so 16-bit storage
-	// stages tiles of whole lines through LDS with 16-byte accesses instead (measured on the 64-instance
-	// batch: BF16 x sweep 3.07 -> 2.4 ms, y/z sweeps faster as well); FP32 keeps register blocks
-	ax.parts[1].staged = b->storage != BFVM_STORAGE_F32;
 	for (size_t i = 0; i < b->ctx.size(); i++)
 	{
 		bfvm_ctx* ctx = b->ctx[i];
@@ -1969,26 +1945,12 @@ void prepare_batch_sweep(bfvm_batch* b, int axis)
 		g.rounding = b->rounding;
 		args[w][i] = make_part_args(g, p, coef, red, A);
 		auto& part = ax.parts[w];
-		if (part.staged)
-		{
-			partition q = p;
-			q.vec = 1;
-			tiles[i] = make_tile_args(ctx, axis, g, q, coef, red, A, batch_tile_bytes, batch_tile_threads);
-			part.lds = std::max(part.lds, tile_lds_bytes(tiles[i], S, ctx->storage));
-			part.threads = std::max(part.threads, tile_threads(tiles[i]));
-			const int64_t per_item = std::max<int64_t>(1, 65536 / std::max<int64_t>(1, tiles[i].tile_elems));
-			for (int64_t t0 = 0; t0 < tiles[i].tiles; t0 += per_item)
-				items[w].push_back(batch_item { (int32_t)i, 0, t0, std::min(tiles[i].tiles, t0 + per_item) });
-		}
-		else
-		{
-			part.lds = std::max(part.lds, partition_lds_bytes(p, S, p.lpg, V > 1 && axis != 0));
-			part.threads = std::max(part.threads, p.threads);
-			const int64_t per_group = (int64_t)p.lpg * n * V;
-			const int64_t per_item = std::max<int64_t>(1, 65536 / per_group);
-			for (int64_t g0 = 0; g0 < args[w][i].groups; g0 += per_item)
-				items[w].push_back(batch_item { (int32_t)i, 0, g0, std::min(args[w][i].groups, g0 + per_item) });
-		}
+		part.lds = std::max(part.lds, partition_lds_bytes(p, S, p.lpg, V > 1 && axis != 0));
+		part.threads = std::max(part.threads, p.threads);
+		const int64_t per_group = (int64_t)p.lpg * n * V;
+		const int64_t per_item = std::max<int64_t>(1, 65536 / per_group);
+		for (int64_t g0 = 0; g0 < args[w][i].groups; g0 += per_item)
+			items[w].push_back(batch_item { (int32_t)i, 0, g0, std::min(args[w][i].groups, g0 + per_item) });
 		ax.any = true;
 	}
 	for (int w = 0; w < 2; w++)
@@ -1998,16 +1960,8 @@ void prepare_batch_sweep(bfvm_batch* b, int axis)
 			continue;
 		if (part.lds > 64 * 1024)
 			throw std::runtime_error("batched sweep: coefficient tables exceed 64 KB of LDS");
-		if (part.staged)
-		{
-			BFVM_HIP(hipMalloc(&part.inst, sizeof(tile_args) * tiles.size()));
-			BFVM_HIP(hipMemcpy(part.inst, tiles.data(), sizeof(tile_args) * tiles.size(), hipMemcpyHostToDevice));
-		}
-		else
-		{
-			BFVM_HIP(hipMalloc(&part.inst, sizeof(part_args) * args[w].size()));
-			BFVM_HIP(hipMemcpy(part.inst, args[w].data(), sizeof(part_args) * args[w].size(), hipMemcpyHostToDevice));
-		}
+		BFVM_HIP(hipMalloc(&part.inst, sizeof(part_args) * args[w].size()));
+		BFVM_HIP(hipMemcpy(part.inst, args[w].data(), sizeof(part_args) * args[w].size(), hipMemcpyHostToDevice));
 		BFVM_HIP(hipMalloc(&part.items, sizeof(batch_item) * items[w].size()));
 		BFVM_HIP(hipMemcpy(part.items, items[w].data(), sizeof(batch_item) * items[w].size(), hipMemcpyHostToDevice));
 		part.nitems = (int)items[w].size();
@@ -2024,10 +1978,7 @@ static void launch_batched_t(bfvm_batch* b)
 	if (p4.nitems)
 		hipLaunchKernelGGL((sweep_batched_kernel<AXIS, K, 4>), dim3(p4.blocks), dim3(p4.threads), p4.lds, b->stream,
 						   (const part_args*)p4.inst, p4.items, p4.nitems);
-	if (p1.nitems && p1.staged)
-		hipLaunchKernelGGL((sweep_batched_staged_kernel<AXIS, K>), dim3(p1.blocks), dim3(p1.threads), p1.lds,
-						   b->stream, (const tile_args*)p1.inst, p1.items, p1.nitems);
-	else if (p1.nitems)
+	if (p1.nitems)
 		hipLaunchKernelGGL((sweep_batched_kernel<AXIS, K, 1>), dim3(p1.blocks), dim3(p1.threads), p1.lds, b->stream,
 						   (const part_args*)p1.inst, p1.items, p1.nitems);
 	BFVM_HIP(hipGetLastError());

@@ -859,9 +859,9 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 // sweep_groups for y/z lines whose threads hold V consecutive flattened plane positions that may wrap over
 // the substrates (V need not divide S): per-substrate tables in LDS extended periodically to S + V - 1 entries
 // and read component by component, per-component substrate and voxel coordinates for the faces.
-template <int M, int AXIS, int V, int K, bool WRAP = true>
-__device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* lds, int64_t g_first, int64_t g_end,
-											 int64_t g_step)
+template <int M, int AXIS, int V, int K>
+__device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* lds, int64_t g_first,
+													 int64_t g_end, int64_t g_step)
 {
 	// this path is held to a tolerance, not to the reference's operation order: multiply-adds may fuse
 #pragma clang fp contract(fast)
@@ -870,7 +870,7 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 	const int S = g.S;
 	const int P = a.P;
 	const int lpg = a.lines_per_group;
-	const int Sp = WRAP ? S + V - 1 : S; // LDS row length of the per-substrate tables
+	const int Sp = S + V - 1; // LDS row length of the per-substrate tables
 	const int ncoef = 4 * a.ntypes * a.mb * Sp;
 	const int nred = reduced_table_floats(P, S);
 	float* lcoef = lds;
@@ -881,18 +881,10 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 	float* lscale = linv + Sp;
 	vec* rv = (vec*)(lds + part_rows_offset(a.ntypes, a.mb, P, S, Sp)); // [2P][lpg]
 
-	if constexpr (WRAP)
+	for (int i = threadIdx.x; i < ncoef; i += blockDim.x)
 	{
-		for (int i = threadIdx.x; i < ncoef; i += blockDim.x)
-		{
-			const int row = i / Sp;
-			lcoef[i] = a.coef[row * S + (i - row * Sp) % S];
-		}
-	}
-	else
-	{
-		for (int i = threadIdx.x; i < ncoef; i += blockDim.x)
-			lcoef[i] = a.coef[i];
+		const int row = i / Sp;
+		lcoef[i] = a.coef[row * S + (i - row * Sp) % S];
 	}
 	for (int i = threadIdx.x; i < nred; i += blockDim.x)
 		lred[i] = a.red[i];
@@ -903,23 +895,17 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 		linv[i] = K == store_f32 ? 1.f : g.inv[si];
 		lscale[i] = K == store_f32 ? 1.f : g.scale[si];
 	}
-	// V table entries from substrate s0 on: one vector read, or component by component where wrapping makes the
-	// start unaligned
+	// V table entries from substrate s0 on, component by component (wrapping leaves the start unaligned)
 	auto cld = [](const float* p) -> vec {
-		if constexpr (WRAP)
-		{
-			vec r;
+		vec r;
 #pragma unroll
-			for (int c = 0; c < V; c++)
-				set_comp(r, c, p[c]);
-			return r;
-		}
-		else
-			return vload<V>(p);
+		for (int c = 0; c < V; c++)
+			set_comp(r, c, p[c]);
+		return r;
 	};
 
 	int line_local, k;
-	thread_slot<AXIS>(threadIdx.x, AXIS == 0 ? S / V : 1, P, lpg, line_local, k);
+	thread_slot<AXIS>(threadIdx.x, 1, P, lpg, line_local, k);
 	const int64_t stride = line_stride<AXIS>(g);
 	const int grid[3] = { g.X, g.Y, g.Z };
 	const size_t fstride = (size_t)a.ntypes * a.mb * Sp;
@@ -1033,7 +1019,7 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 			const int ll = pr / V, c = pr - ll * V;
 			if (group * lpg + ll < a.lines)
 			{
-				const int sc = WRAP ? (group_s + ll * V + c) % S : (group_s + ll * V) % S + c;
+				const int sc = (group_s + ll * V + c) % S;
 				twisted_solve((const float4*)lred + sc, lred[8 * P * S + sc], S, (float*)rv + ll * V + c, lpg * V, P,
 							  threadIdx.x & 1);
 			}
@@ -1049,35 +1035,25 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 			// only lines on a face of another axis and the end blocks of a line can meet a face: the
 			// conditions are read for those alone
 			bool near_face = start == 0 || start + mk == a.n;
-			if constexpr (WRAP)
+			for (int c = 0; c < V; c++)
 			{
-				for (int c = 0; c < V; c++)
-				{
-					int sc, cc[3];
-					comp_coords<AXIS, WRAP>(g, coord, s0, c, sc, cc);
-					for (int d = 0; d < g.dims; d++)
-						if (d != AXIS)
-							near_face |= cc[d] == 0 || cc[d] == grid[d] - 1;
-				}
-			}
-			else
-			{
+				int sc, cc[3];
+				comp_coords<AXIS, true>(g, coord, s0, c, sc, cc);
 				for (int d = 0; d < g.dims; d++)
 					if (d != AXIS)
-						near_face |= coord[d] == 0 || coord[d] == grid[d] - 1;
+						near_face |= cc[d] == 0 || cc[d] == grid[d] - 1;
 			}
 			if (g.fcond && near_face)
 			{
 				for (int c = 0; c < V; c++)
 				{
-					int sc = s0 + c, cc[3];
-					if constexpr (WRAP)
-						comp_coords<AXIS, WRAP>(g, coord, s0, c, sc, cc);
+					int sc, cc[3];
+					comp_coords<AXIS, true>(g, coord, s0, c, sc, cc);
 					for (int d = 0; d < g.dims; d++)
 					{
 						const bool cmin = g.fcond[(2 * d) * S + sc] != 0;
 						const bool cmax = g.fcond[(2 * d + 1) * S + sc] != 0;
-						const int cd = WRAP ? cc[d] : coord[d];
+						const int cd = cc[d];
 						if (d == AXIS)
 						{
 							end_min |= cmin;
@@ -1103,7 +1079,7 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 						x = v[j] - cld(ca + j * Sp) * u0 - cld(cgp + j * Sp) * u1;
 					const int i = start + j;
 					if (line_face || (i == 0 && end_min) || (i == a.n - 1 && end_max))
-						x = faces_slow<AXIS, V, WRAP, K>(x, g, s0, coord, i, sc);
+						x = faces_slow<AXIS, V, true, K>(x, g, s0, coord, i, sc);
 					dstore_sr<K, V>(g.data, off0 + j * stride, x, key);
 				}
 			}

@@ -176,3 +176,27 @@ def test_batch_vector_widths_with_dirichlet(pkg, oracle_mod):
     batch, oracles, solvers = run_batch(pkg, oracle_mod, cfgs, 6)
     check_instances(cfgs, oracles, solvers, 6)
     batch.close()
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_batch_vector_widths_with_dirichlet_16bit(storage, pkg, oracle_mod):
+    """The wrapping and one-element layouts of the previous test in 16-bit storage, where the solves run on the
+    stored (scaled) values and only Dirichlet values take the scale: every instance within the storage form's
+    error of the FP32 oracle (relative L2: FP16 1e-3, BF16 1e-2)."""
+    from parabiofvm_amd.configs import make_config
+
+    rng = np.random.default_rng(19)
+    shapes = [((36, 20, 18), 3), ((10, 9, 21), 5), ((9, 11, 7), 3), ((8, 6, 19), 6), ((16, 5, 6), 1),
+              ((7, 18, 9), 4)]
+    cfgs = []
+    for k, (grid, S) in enumerate(shapes):
+        c = make_config(1, seed=200 + k, shape=grid, substrates=S, agents=60)
+        c["faces"] = {key: (vals, (rng.random(S) < 0.7).astype(np.uint8)) for key, (vals, _) in c["faces"].items()}
+        cfgs.append(c)
+    batch, oracles, solvers = run_batch(pkg, oracle_mod, cfgs, 6, storage=storage)
+    bound = 1e-3 if storage == "f16" else 1e-2
+    for k, (cfg, o, g) in enumerate(zip(cfgs, oracles, solvers)):
+        d = g.densities.astype(np.float64) - o.densities
+        rel = np.linalg.norm(d) / max(np.linalg.norm(o.densities.astype(np.float64)), 1e-30)
+        assert rel <= bound, f"instance {k} grid {cfg['grid']} S {cfg['S']}: rel L2 {rel:.2e}"
+    batch.close()

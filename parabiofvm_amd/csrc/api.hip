@@ -1121,6 +1121,7 @@ static void free_batch(bfvm_batch* b)
 	for (int a = 0; a < 3; a++)
 		free_batch_sweep(b, a);
 	free_batch_gradient(b);
+	free_batch_cell(b);
 	if (b->rounding)
 		(void)hipFree(b->rounding);
 	b->rounding = nullptr;
@@ -1186,13 +1187,24 @@ static void enqueue_batch_gradient(bfvm_batch* b)
 	}
 }
 
-static void enqueue_batch_step(bfvm_batch* b, bool recompute, bool with_gradient)
+// the secretion of every instance: batched launches when the plan allows, else instance by instance
+static void enqueue_batch_cell(bfvm_batch* b, bool recompute)
 {
+	if (b->cell.batched)
+	{
+		launch_batched_cell(b, recompute);
+		return;
+	}
 	for (bfvm_ctx* c : b->ctx)
 	{
 		batch_override o(c, b);
 		launch_cell(c, recompute);
 	}
+}
+
+static void enqueue_batch_step(bfvm_batch* b, bool recompute, bool with_gradient)
+{
+	enqueue_batch_cell(b, recompute);
 	enqueue_batch_sweeps(b);
 	if (with_gradient)
 		enqueue_batch_gradient(b);
@@ -1274,6 +1286,7 @@ int bfvm_batch_timestep(bfvm_batch* batch, int32_t recompute, int32_t with_gradi
 		BFVM_HIP(hipSetDevice(batch->device));
 		auto& slot = batch->graphs[(recompute ? 2 : 0) + (with_gradient ? 1 : 0)];
 		const uint64_t epoch = batch_epoch(batch);
+		refresh_batch_cell(batch, epoch);
 		if (!slot.exec || slot.epoch != epoch)
 		{
 			if (slot.exec)
@@ -1324,6 +1337,7 @@ int bfvm_batch_benchmark(bfvm_batch* batch, int32_t what, int32_t iterations, fl
 			for (const bfvm_ctx* c : batch->ctx)
 				check_gradient(c);
 		BFVM_HIP(hipSetDevice(batch->device));
+		refresh_batch_cell(batch, batch_epoch(batch));
 		auto one = [&]() {
 			switch (what)
 			{
@@ -1337,11 +1351,7 @@ int bfvm_batch_benchmark(bfvm_batch* batch, int32_t what, int32_t iterations, fl
 				break;
 			case BFVM_PHASE_SECRETION:
 			case BFVM_PHASE_SECRETION_RECOMPUTE:
-				for (bfvm_ctx* c : batch->ctx)
-				{
-					batch_override o(c, batch);
-					launch_cell(c, what == BFVM_PHASE_SECRETION_RECOMPUTE);
-				}
+				enqueue_batch_cell(batch, what == BFVM_PHASE_SECRETION_RECOMPUTE);
 				break;
 			case BFVM_PHASE_DIFFUSION:
 				enqueue_batch_sweeps(batch);

@@ -330,6 +330,20 @@ struct batch_item
 	int64_t begin, end;
 };
 
+// Per-instance arguments of the batched secretion kernels (one instance's agents and buffers).
+struct cell_args
+{
+	int A, S, with_internalized, voxel_volume;
+	float dt;
+	const int64_t* vox;
+	const float *vol, *sec, *sat, *upt, *nex;
+	float *num, *den, *fac, *rnum, *rden, *rfac, *intern;
+	int *ballots, *win, *conflict;
+	void* dens;
+	const float *scale, *inv;
+	const unsigned* rounding;
+};
+
 constexpr int batch_chunk = 16; // register block of the staged and batched sweeps (one line per thread)
 constexpr int batch_group_threads = 256; // workgroup size the batched sweeps aim for
 constexpr int batch_max_threads = 1024;
@@ -431,6 +445,19 @@ struct bfvm_batch
 		int blocks = 0;
 		std::vector<int> single; // instances of fewer than 3 dimensions (per-instance launch)
 	} grad;
+	// batched secretion: per-instance arguments and work items over (agent, substrate) and agent blocks, rebuilt
+	// when an instance's agents change (cell_epoch); per-instance launches when an instance uses the sorted
+	// reduction
+	struct cell_plan
+	{
+		bfvm::cell_args* inst = nullptr;
+		bfvm::batch_item* pair_items = nullptr;
+		bfvm::batch_item* agent_items = nullptr;
+		size_t pair_cap = 0, agent_cap = 0;
+		int npair = 0, nagent = 0;
+		bool batched = false;
+		uint64_t epoch = ~0ull;
+	} cell;
 	uint64_t epoch = 0;
 	bfvm::graph_slot graphs[4];
 };
@@ -455,6 +482,9 @@ void launch_max_abs(bfvm_ctx* ctx, const float* src, int64_t n, unsigned* out);
 void launch_advance_rounding(bfvm_ctx* ctx);
 
 void launch_cell(bfvm_ctx* ctx, bool recompute);
+void refresh_batch_cell(bfvm_batch* b, uint64_t epoch); // host-synchronous: call outside stream capture
+void launch_batched_cell(bfvm_batch* b, bool recompute);
+void free_batch_cell(bfvm_batch* b);
 void launch_bulk(bfvm_ctx* ctx);
 void launch_cell_voxels(bfvm_ctx* ctx);
 void launch_cell_sort(bfvm_ctx* ctx); // sorted reduction: order the agents by voxel (upload time)

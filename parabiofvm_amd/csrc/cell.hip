@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void cell_voxels_kernel(cell_geom g, int A, co
 
 // recompute, part 1 (host/cell_solver.cpp:35-85): intermediates and reduced-slot reset per (agent, s);
 // ballot reset per agent.
-__global__ __launch_bounds__(256) void cell_prepare_kernel(int A, int S, float voxel_volume, float dt,
+__device__ __forceinline__ void cell_prepare_at(int64_t k, int A, int S, float voxel_volume, float dt,
 														   const int64_t* __restrict__ vox,
 														   const float* __restrict__ vol, const float* __restrict__ sec,
 														   const float* __restrict__ sat, const float* __restrict__ upt,
@@ -68,7 +68,6 @@ __global__ __launch_bounds__(256) void cell_prepare_kernel(int A, int S, float v
 														   float* __restrict__ rfac, int* __restrict__ ballots,
 														   int* __restrict__ conflict)
 {
-	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k == 0)
 		atomicExch(conflict, 0);
 	if (k >= (int64_t)A * S)
@@ -90,12 +89,25 @@ __global__ __launch_bounds__(256) void cell_prepare_kernel(int A, int S, float v
 	rfac[k] = 0;
 }
 
+__global__ __launch_bounds__(256) void cell_prepare_kernel(int A, int S, float voxel_volume, float dt,
+														   const int64_t* __restrict__ vox,
+														   const float* __restrict__ vol, const float* __restrict__ sec,
+														   const float* __restrict__ sat, const float* __restrict__ upt,
+														   const float* __restrict__ nex, float* __restrict__ num,
+														   float* __restrict__ den, float* __restrict__ fac,
+														   float* __restrict__ rnum, float* __restrict__ rden,
+														   float* __restrict__ rfac, int* __restrict__ ballots,
+														   int* __restrict__ conflict)
+{
+	cell_prepare_at((int64_t)blockIdx.x * blockDim.x + threadIdx.x, A, S, voxel_volume, dt, vox, vol, sec, sat, upt,
+					nex, num, den, fac, rnum, rden, rfac, ballots, conflict);
+}
+
 // recompute, part 2 (host/cell_solver.cpp:98-122): claim the voxel; win[i] = the agent whose slot collects i
-__global__ __launch_bounds__(256) void cell_ballot_kernel(int A, const int64_t* __restrict__ vox,
+__device__ __forceinline__ void cell_ballot_at(int64_t i, int A, const int64_t* __restrict__ vox,
 														  int* __restrict__ ballots, int* __restrict__ win,
 														  int* __restrict__ conflict)
 {
-	const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= A)
 		return;
 	const int64_t v = vox[i];
@@ -114,14 +126,20 @@ __global__ __launch_bounds__(256) void cell_ballot_kernel(int A, const int64_t* 
 	}
 }
 
+__global__ __launch_bounds__(256) void cell_ballot_kernel(int A, const int64_t* __restrict__ vox,
+														  int* __restrict__ ballots, int* __restrict__ win,
+														  int* __restrict__ conflict)
+{
+	cell_ballot_at((int64_t)blockIdx.x * blockDim.x + threadIdx.x, A, vox, ballots, win, conflict);
+}
+
 // recompute, part 3 (host/cell_solver.cpp:108-133): every (agent, s) adds into its winner's slot; the
 // winner adds den + 1.
-__global__ __launch_bounds__(256) void cell_sum_kernel(int A, int S, const int* __restrict__ win,
+__device__ __forceinline__ void cell_sum_at(int64_t k, int A, int S, const int* __restrict__ win,
 													   const float* __restrict__ num, const float* __restrict__ den,
 													   const float* __restrict__ fac, float* __restrict__ rnum,
 													   float* __restrict__ rden, float* __restrict__ rfac)
 {
-	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= (int64_t)A * S)
 		return;
 	const int64_t i = k / S;
@@ -133,6 +151,14 @@ __global__ __launch_bounds__(256) void cell_sum_kernel(int A, int S, const int* 
 	atomicAdd(&rnum[t], num[k]);
 	atomicAdd(&rden[t], w == (int)i ? den[k] + 1 : den[k]);
 	atomicAdd(&rfac[t], fac[k]);
+}
+
+__global__ __launch_bounds__(256) void cell_sum_kernel(int A, int S, const int* __restrict__ win,
+													   const float* __restrict__ num, const float* __restrict__ den,
+													   const float* __restrict__ fac, float* __restrict__ rnum,
+													   float* __restrict__ rden, float* __restrict__ rfac)
+{
+	cell_sum_at((int64_t)blockIdx.x * blockDim.x + threadIdx.x, A, S, win, num, den, fac, rnum, rden, rfac);
 }
 
 // Sorted reduction.  At upload the agents are radix-sorted by voxel (stable, so a voxel's agents keep
@@ -216,7 +242,7 @@ __global__ __launch_bounds__(256) void cell_sorted_sum_kernel(int A, int S, uint
 // No conflict and internalized: the fused update of I and D.  Conflict and internalized: I with the
 // agent's own terms and the current D (D is updated by part 2 after the kernel boundary).
 template <int K>
-__global__ __launch_bounds__(256) void cell_apply_kernel(int A, int S, int with_internalized, int voxel_volume,
+__device__ __forceinline__ void cell_apply_at(int64_t k, int A, int S, int with_internalized, int voxel_volume,
 														 const int64_t* __restrict__ vox, void* __restrict__ dens,
 														 const float* __restrict__ scale, const float* __restrict__ inv,
 														 const unsigned* __restrict__ rounding,
@@ -229,7 +255,6 @@ __global__ __launch_bounds__(256) void cell_apply_kernel(int A, int S, int with_
 {
 	if (!with_internalized)
 		return;
-	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= (int64_t)A * S)
 		return;
 	const int64_t i = k / S;
@@ -251,10 +276,26 @@ __global__ __launch_bounds__(256) void cell_apply_kernel(int A, int S, int with_
 		intern[k] = intern[k] - vv * (((-den[k]) * d + num[k]) / (1 + den[k]) + fac[k]);
 }
 
+template <int K>
+__global__ __launch_bounds__(256) void cell_apply_kernel(int A, int S, int with_internalized, int voxel_volume,
+														 const int64_t* __restrict__ vox, void* __restrict__ dens,
+														 const float* __restrict__ scale, const float* __restrict__ inv,
+														 const unsigned* __restrict__ rounding,
+														 float* __restrict__ intern, const float* __restrict__ num,
+														 const float* __restrict__ den, const float* __restrict__ fac,
+														 const float* __restrict__ rnum,
+														 const float* __restrict__ rden,
+														 const float* __restrict__ rfac,
+														 const int* __restrict__ conflict)
+{
+	cell_apply_at<K>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, A, S, with_internalized, voxel_volume, vox, dens,
+					 scale, inv, rounding, intern, num, den, fac, rnum, rden, rfac, conflict);
+}
+
 // Apply, part 2 (host/cell_solver.cpp:154-171, 236-244): the ballot winner of each voxel updates D with
 // the reduced sums.  Skipped when part 1 already did the fused update.
 template <int K>
-__global__ __launch_bounds__(256) void cell_densities_kernel(int A, int S, int with_internalized,
+__device__ __forceinline__ void cell_densities_at(int64_t k, int A, int S, int with_internalized,
 															 const int64_t* __restrict__ vox, void* __restrict__ dens,
 															 const float* __restrict__ scale,
 															 const float* __restrict__ inv,
@@ -267,7 +308,6 @@ __global__ __launch_bounds__(256) void cell_densities_kernel(int A, int S, int w
 {
 	if (with_internalized && *conflict == 0)
 		return;
-	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= (int64_t)A * S)
 		return;
 	const int64_t i = k / S;
@@ -281,6 +321,62 @@ __global__ __launch_bounds__(256) void cell_densities_kernel(int A, int S, int w
 	const float d = K == store_f32 ? dload<K, 1>(dens, at) : dload<K, 1>(dens, at) * inv[s];
 	const float nd = (d + rnum[k]) / rden[k] + rfac[k];
 	dstore_sr<K, 1>(dens, at, K == store_f32 ? nd : nd * scale[s], K == store_f32 ? 0u : rounding_key(rounding, 6));
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void cell_densities_kernel(int A, int S, int with_internalized,
+															 const int64_t* __restrict__ vox, void* __restrict__ dens,
+															 const float* __restrict__ scale,
+															 const float* __restrict__ inv,
+															 const unsigned* __restrict__ rounding,
+															 const float* __restrict__ rnum,
+															 const float* __restrict__ rden,
+															 const float* __restrict__ rfac,
+															 const int* __restrict__ ballots,
+															 const int* __restrict__ conflict)
+{
+	cell_densities_at<K>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, A, S, with_internalized, vox, dens, scale, inv,
+						 rounding, rnum, rden, rfac, ballots, conflict);
+}
+
+// Many microenvironments (bfvm_batch): one launch per secretion phase over every instance, workgroups walking
+// a work list of (instance, block range) items; each block runs the phase's per-element function for its 256
+// elements, as the single-instance kernels do.
+enum cell_phase
+{
+	phase_prepare,
+	phase_ballot,
+	phase_sum,
+	phase_apply,
+	phase_densities
+};
+
+template <int K, int PHASE>
+__global__ __launch_bounds__(256) void cell_batched_kernel(const cell_args* __restrict__ inst,
+														   const batch_item* __restrict__ items, int nitems)
+{
+	for (int it = blockIdx.x; it < nitems; it += gridDim.x)
+	{
+		const batch_item w = items[it];
+		const cell_args c = inst[w.instance];
+		for (int64_t blk = w.begin; blk < w.end; blk++)
+		{
+			const int64_t k = blk * blockDim.x + threadIdx.x;
+			if constexpr (PHASE == phase_prepare)
+				cell_prepare_at(k, c.A, c.S, (float)c.voxel_volume, c.dt, c.vox, c.vol, c.sec, c.sat, c.upt, c.nex, c.num,
+								c.den, c.fac, c.rnum, c.rden, c.rfac, c.ballots, c.conflict);
+			else if constexpr (PHASE == phase_ballot)
+				cell_ballot_at(k, c.A, c.vox, c.ballots, c.win, c.conflict);
+			else if constexpr (PHASE == phase_sum)
+				cell_sum_at(k, c.A, c.S, c.win, c.num, c.den, c.fac, c.rnum, c.rden, c.rfac);
+			else if constexpr (PHASE == phase_apply)
+				cell_apply_at<K>(k, c.A, c.S, c.with_internalized, c.voxel_volume, c.vox, c.dens, c.scale, c.inv,
+								 c.rounding, c.intern, c.num, c.den, c.fac, c.rnum, c.rden, c.rfac, c.conflict);
+			else
+				cell_densities_at<K>(k, c.A, c.S, c.with_internalized, c.vox, c.dens, c.scale, c.inv, c.rounding,
+									 c.rnum, c.rden, c.rfac, c.ballots, c.conflict);
+		}
+	}
 }
 
 // host/cell_solver.cpp:301-328: D += I*F/vv, I = 0 for one agent.
@@ -430,6 +526,125 @@ void launch_cell(bfvm_ctx* ctx, bool recompute)
 						   a.rnum, a.rden,
 						   a.rfac, ctx->ballots, ctx->conflict);
 		BFVM_HIP(hipGetLastError());
+	});
+}
+
+void free_batch_cell(bfvm_batch* b)
+{
+	auto& cp = b->cell;
+	if (cp.inst)
+		BFVM_HIP(hipFree(cp.inst));
+	if (cp.pair_items)
+		BFVM_HIP(hipFree(cp.pair_items));
+	if (cp.agent_items)
+		BFVM_HIP(hipFree(cp.agent_items));
+	cp = bfvm_batch::cell_plan {};
+}
+
+// Per-instance arguments and work items of the batched secretion, rebuilt when `epoch` (the batch's sum of
+// instance epochs, which moves whenever an instance's agents change) differs from the plan's.  Items cover
+// blocks of 256 (agent, substrate) pairs (at least one block per instance, so the recompute clears every
+// conflict flag) and blocks of 256 agents, 8 blocks per item.
+void refresh_batch_cell(bfvm_batch* b, uint64_t epoch)
+{
+	auto& cp = b->cell;
+	if (cp.epoch == epoch)
+		return;
+	cp.batched = true;
+	for (const bfvm_ctx* c : b->ctx)
+		cp.batched &= !c->ag.sorted;
+	if (!cp.batched)
+	{
+		cp.epoch = epoch;
+		return;
+	}
+	constexpr int64_t per_item = 8;
+	std::vector<cell_args> args(b->ctx.size());
+	std::vector<batch_item> pairs, agents;
+	for (size_t i = 0; i < b->ctx.size(); i++)
+	{
+		const bfvm_ctx* ctx = b->ctx[i];
+		const auto& a = ctx->ag;
+		cell_args& c = args[i];
+		c.A = a.count;
+		c.S = ctx->mesh.S;
+		c.with_internalized = ctx->compute_internalized ? 1 : 0;
+		c.voxel_volume = voxel_volume(ctx);
+		c.dt = ctx->mesh.dt;
+		c.vox = a.vox;
+		c.vol = a.vol;
+		c.sec = a.sec;
+		c.sat = a.sat;
+		c.upt = a.upt;
+		c.nex = a.nex;
+		c.num = a.num;
+		c.den = a.den;
+		c.fac = a.fac;
+		c.rnum = a.rnum;
+		c.rden = a.rden;
+		c.rfac = a.rfac;
+		c.intern = a.intern;
+		c.ballots = ctx->ballots;
+		c.win = a.win;
+		c.conflict = ctx->conflict;
+		c.dens = ctx->dens;
+		c.scale = ctx->dscale;
+		c.inv = ctx->dinv;
+		c.rounding = b->rounding;
+		const int64_t pair_blocks = std::max<int64_t>(1, ((int64_t)c.A * c.S + 255) / 256);
+		const int64_t agent_blocks = ((int64_t)c.A + 255) / 256;
+		for (int64_t b0 = 0; b0 < pair_blocks; b0 += per_item)
+			pairs.push_back(batch_item { (int32_t)i, 0, b0, std::min(pair_blocks, b0 + per_item) });
+		for (int64_t b0 = 0; b0 < agent_blocks; b0 += per_item)
+			agents.push_back(batch_item { (int32_t)i, 0, b0, std::min(agent_blocks, b0 + per_item) });
+	}
+	if (!cp.inst)
+		BFVM_HIP(hipMalloc(&cp.inst, sizeof(cell_args) * args.size()));
+	BFVM_HIP(hipMemcpy(cp.inst, args.data(), sizeof(cell_args) * args.size(), hipMemcpyHostToDevice));
+	auto upload = [](const std::vector<batch_item>& v, batch_item*& dst, size_t& cap) {
+		if (v.size() > cap)
+		{
+			if (dst)
+				BFVM_HIP(hipFree(dst));
+			dst = nullptr;
+			BFVM_HIP(hipMalloc(&dst, sizeof(batch_item) * v.size()));
+			cap = v.size();
+		}
+		if (!v.empty())
+			BFVM_HIP(hipMemcpy(dst, v.data(), sizeof(batch_item) * v.size(), hipMemcpyHostToDevice));
+	};
+	upload(pairs, cp.pair_items, cp.pair_cap);
+	upload(agents, cp.agent_items, cp.agent_cap);
+	cp.npair = (int)pairs.size();
+	cp.nagent = (int)agents.size();
+	cp.epoch = epoch;
+}
+
+template <int K, int PHASE>
+static void launch_cell_phase(bfvm_batch* b, const batch_item* items, int n)
+{
+	if (n == 0)
+		return;
+	hipLaunchKernelGGL((cell_batched_kernel<K, PHASE>), dim3(std::min(n, 4096)), dim3(256), 0, b->stream,
+					   (const cell_args*)b->cell.inst, items, n);
+	BFVM_HIP(hipGetLastError());
+}
+
+// The secretion of every instance of a refreshed, batched plan: recompute (prepare, ballot, sum) and apply
+// (apply, densities), one launch per phase, in the order of launch_cell.
+void launch_batched_cell(bfvm_batch* b, bool recompute)
+{
+	const auto& cp = b->cell;
+	by_storage(b->storage, [&](auto kind) {
+		constexpr int K = decltype(kind)::value;
+		if (recompute)
+		{
+			launch_cell_phase<K, phase_prepare>(b, cp.pair_items, cp.npair);
+			launch_cell_phase<K, phase_ballot>(b, cp.agent_items, cp.nagent);
+			launch_cell_phase<K, phase_sum>(b, cp.pair_items, cp.npair);
+		}
+		launch_cell_phase<K, phase_apply>(b, cp.pair_items, cp.npair);
+		launch_cell_phase<K, phase_densities>(b, cp.pair_items, cp.npair);
 	});
 }
 

@@ -80,12 +80,12 @@ constexpr int max_types = 4;
 // floats of the reduced-system table of a line of P blocks
 __host__ __device__ constexpr int reduced_table_floats(int P, int S) { return 4 * 2 * P * S + S; }
 // LDS of a partitioned sweep workgroup, in floats: the coefficient table [4][ntypes][mb][Sp], the reduced-system
-// table, A, 1/scale and scale [Sp] each, then (from the returned offset, 16-byte aligned) the reduced rows.  Sp is
+// table, A and the storage scale [Sp] each, then (from the returned offset, 16-byte aligned) the reduced rows.  Sp is
 // S, or S + V - 1 when substrate vectors wrap (rows extended periodically so V consecutive entries from any
 // substrate are the wrapped substrates).
 __host__ __device__ constexpr int part_rows_offset(int ntypes, int mb, int P, int S, int Sp)
 {
-	return (4 * ntypes * mb * Sp + reduced_table_floats(P, S) + 3 * Sp + 3) / 4 * 4;
+	return (4 * ntypes * mb * Sp + reduced_table_floats(P, S) + 2 * Sp + 3) / 4 * 4;
 }
 
 __global__ void partition_coefficients_kernel(float* __restrict__ coef, float* __restrict__ red,
@@ -662,10 +662,9 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 	const int nred = reduced_table_floats(P, S);
 	float* lcoef = lds;
 	float* lred = lds + ncoef;
-	// per-substrate A, 1/scale and scale, so the group loop reads no global memory besides the densities
+	// per-substrate A and storage scale, so the group loop reads no global memory besides the densities
 	float* lA = lred + nred;
-	float* linv = lA + S;
-	float* lscale = linv + S;
+	float* lscale = lA + S;
 	vec* rv = (vec*)(lds + part_rows_offset(a.ntypes, a.mb, P, S, S)); // [2P][lpg]
 
 	for (int i = threadIdx.x; i < ncoef; i += blockDim.x)
@@ -675,7 +674,6 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 	for (int i = threadIdx.x; i < S; i += blockDim.x)
 	{
 		lA[i] = a.A[i];
-		linv[i] = K == store_f32 ? 1.f : g.inv[i];
 		lscale[i] = K == store_f32 ? 1.f : g.scale[i];
 	}
 
@@ -875,10 +873,9 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 	const int nred = reduced_table_floats(P, S);
 	float* lcoef = lds;
 	float* lred = lds + ncoef;
-	// per-substrate A, 1/scale and scale, so the group loop reads no global memory besides the densities
+	// per-substrate A and storage scale, so the group loop reads no global memory besides the densities
 	float* lA = lred + nred;
-	float* linv = lA + Sp;
-	float* lscale = linv + Sp;
+	float* lscale = lA + Sp;
 	vec* rv = (vec*)(lds + part_rows_offset(a.ntypes, a.mb, P, S, Sp)); // [2P][lpg]
 
 	for (int i = threadIdx.x; i < ncoef; i += blockDim.x)
@@ -892,7 +889,6 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 	{
 		const int si = i % S;
 		lA[i] = a.A[si];
-		linv[i] = K == store_f32 ? 1.f : g.inv[si];
 		lscale[i] = K == store_f32 ? 1.f : g.scale[si];
 	}
 	// V table entries from substrate s0 on, component by component (wrapping leaves the start unaligned)

@@ -440,9 +440,10 @@ struct bfvm_batch
 	struct gradient_plan
 	{
 		void* inst = nullptr; // device gradient_args per instance
-		bfvm::batch_item* items = nullptr;
-		int nitems = 0;
-		int blocks = 0;
+		// work items per thread layout: one element, 4 substrates of a voxel, and all of them
+		bfvm::batch_item* items[3] = { nullptr, nullptr, nullptr };
+		int nitems[3] = { 0, 0, 0 };
+		int blocks[3] = { 0, 0, 0 };
 		std::vector<int> single; // instances of fewer than 3 dimensions (per-instance launch)
 	} grad;
 	// batched secretion: per-instance arguments and work items over (agent, substrate) and agent blocks, rebuilt

@@ -246,10 +246,12 @@ int bfvm_benchmark(bfvm_ctx* ctx, int32_t what, int32_t iterations, float* avg_m
 /*
  * Batches of independent microenvironments.  No counterpart in the reference, which runs one
  * microenvironment per solver (src/solver.h); a batch advances many initialized contexts of any mesh shape
- * and substrate count with ONE launch per sweep axis and one for the gradient, each driven by a device work
- * list of (instance, line-group / tile range) items, and replays the whole step (secretion of every
- * instance -> Dirichlet -> x/y/z sweeps -> gradient) as one captured HIP graph.  Per-instance results equal
- * what bfvm_timestep gives for that context (same kernels, one substrate per thread, 16-element blocks).
+ * and substrate count with work-list launches over all instances: per secretion phase, per sweep axis (one
+ * launch per thread layout: 4-element vectors where 4 divides the row, else one element) and for the
+ * gradient, each driven by a device work list of (instance, block / line-group / tile range) items, and
+ * replays the whole step (secretion of every instance -> Dirichlet -> x/y/z sweeps -> gradient) as one
+ * captured HIP graph.  Per-instance results equal what bfvm_timestep gives for that context to the
+ * partitioned sweep's tolerance (the same algorithm with 16-element blocks).
  *
  * All contexts must be initialized on one device with one storage form and the partitioned sweep; they
  * keep their data (store/load through the per-context calls) and must not be re-initialized or destroyed

@@ -647,7 +647,7 @@ __device__ __forceinline__ void twisted_solve(const float4* tab, float mu, int S
 
 // Groups g_first, g_first + g_step, ... < g_end of one partitioned sweep, by one workgroup.  The caller
 // synchronises the workgroup before reusing `lds`.
-template <int M, int AXIS, int V, int K, bool PF = (K != store_f32 && AXIS != 0)>
+template <int M, int AXIS, int V, int K, bool PF = (K != store_f32 && AXIS != 0), bool NT = (K == store_f32 && AXIS != 0)>
 __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int64_t g_first, int64_t g_end,
 											 int64_t g_step)
 {
@@ -846,7 +846,13 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 					const int i = start + j;
 					if (line_face || (i == 0 && end_min) || (i == a.n - 1 && end_max))
 						x = faces_slow<AXIS, V, false, K>(x, g, s0, coord, i, sc);
-					dstore_sr<K, V>(g.data, off0 + j * stride, x, key);
+					// NT: FP32 y/z outputs of a large instance are streamed past the caches (measured: y 0.964 ->
+					// 0.947 ms, z 1.037 -> 1.023 ms on 400^3 x 8; along x, whose rows stay in cache, and for the
+					// small instances of a batch, whose outputs the next sweep finds in cache, they are slower)
+					if constexpr (NT)
+						__builtin_nontemporal_store(x, (vec*)((float*)g.data + off0 + j * stride));
+					else
+						dstore_sr<K, V>(g.data, off0 + j * stride, x, key);
 				}
 			}
 		}
@@ -1106,7 +1112,7 @@ __global__ __launch_bounds__(V == 1 ? batch_max_threads : 512) void sweep_batche
 		if constexpr (V > 1 && AXIS != 0)
 			sweep_groups_wrapped<batch_chunk, AXIS, V, K>(a, lds, w.begin, w.end, 1);
 		else
-			sweep_groups<batch_chunk, AXIS, V, K, (K != store_f32)>(a, lds, w.begin, w.end, 1);
+			sweep_groups<batch_chunk, AXIS, V, K, (K != store_f32), false>(a, lds, w.begin, w.end, 1);
 	}
 }
 

@@ -181,6 +181,11 @@ int bfvm_set_sweep_algorithm(bfvm_ctx* ctx, int32_t algorithm, int32_t chunk);
  * (y/z: consecutive columns per workgroup, x: rows per workgroup; 0 = automatic).  On a configuration the
  * register/LDS budget cannot hold, the previous one is kept and an error is returned. */
 int bfvm_set_sweep_tuning(bfvm_ctx* ctx, int32_t axis, int32_t chunk, int32_t lines_per_group);
+/* Most workgroups a partitioned or staged sweep launches (0 = automatic: 8192 partitioned, 4096 staged).
+ * Each workgroup loops over line groups (or tiles) in steps of the launched count, so a small cap makes
+ * every workgroup run several passes.  No counterpart in the reference, whose sweeps launch one 256-thread
+ * block per 256 lines (CUDA/diffusion_solver.cu:947-987). */
+int bfvm_set_sweep_workgroups(bfvm_ctx* ctx, int32_t workgroups);
 /* Workgroups the 3-D gradient kernel aims for by splitting z into chunks (0 = automatic). */
 int bfvm_set_gradient_tuning(bfvm_ctx* ctx, int32_t workgroups);
 
@@ -254,9 +259,11 @@ int bfvm_benchmark(bfvm_ctx* ctx, int32_t what, int32_t iterations, float* avg_m
  * partitioned sweep's tolerance (the same algorithm with 16-element blocks).
  *
  * All contexts must be initialized on one device with one storage form and the partitioned sweep; they
- * keep their data (store/load through the per-context calls) and must not be re-initialized or destroyed
- * while the batch exists.  The batch has its own stream: call bfvm_batch_wait before loading results from
- * a context, and bfvm_wait_for_all(ctx) after an asynchronous context call before the next batch step.
+ * keep their data (store/load through the per-context calls).  A context re-initialized while the batch
+ * exists makes the batch's next step fail with BFVM_ERR_STATE; bfvm_destroy refuses a context that belongs to
+ * a batch.  The batch has its own stream, ordered with the contexts' streams both ways: every context call
+ * that touches device data first waits for the steps of its batches, and a batch step first waits for work
+ * that context calls enqueued on the instances' streams.  bfvm_batch_wait waits for the batch's steps.
  */
 typedef struct bfvm_batch bfvm_batch;
 int bfvm_batch_create(bfvm_ctx* const* contexts, int32_t count, bfvm_batch** out);

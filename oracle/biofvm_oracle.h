@@ -19,8 +19,12 @@
  *   - Thomas sweeps: a dense float64 tridiagonal solve of the same LOD system;
  *   - voxel index: exhaustive boundary checks against float32 (p-min)/dx;
  *   - secretion with conflicts: a brute-force per-voxel reduction
- *     (test/utils.cpp:134-202 semantics).
- * Functions without such a pin: none.
+ *     (test/utils.cpp:134-202 semantics);
+ *   - release at death (or_cell_release): its closed form in FP64, D + I*f/vv at
+ *     the agent's voxel and I cleared, a no-op with internalization off and for
+ *     agents outside the mesh (the reference's suites never test release);
+ *   - bulk supply/uptake (or_bulk_solve): its geometric closed forms in FP64.
+ * Every exported compute function has one of these pins.
  */
 #ifndef BIOFVM_ORACLE_H
 #define BIOFVM_ORACLE_H

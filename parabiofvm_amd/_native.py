@@ -110,6 +110,7 @@ PROTOTYPES = {
     "bfvm_sweep_partition": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32, _I32, _I32]),
     "bfvm_set_sweep_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "bfvm_set_gradient_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "bfvm_set_sweep_workgroups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "bfvm_exact_coefficients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p]),
     "bfvm_voxel_positions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),

@@ -1,7 +1,13 @@
-// Builder with validation (include/BioFVM/microenvironment_builder.h:9-72 of the reference).
+// Step-by-step construction of a microenvironment with validation.  Public interface of the reference's
+// builder (include/BioFVM/microenvironment_builder.h:9-72), so that setup code written against it compiles
+// unchanged; the bookkeeping behind it is this facade's own: one record per substrate and one per interior
+// Dirichlet node, flattened into the microenvironment's arrays only by build().
 #pragma once
 
+#include <array>
 #include <optional>
+#include <string>
+#include <vector>
 
 #include "mesh.h"
 #include "microenvironment.h"
@@ -11,26 +17,29 @@ namespace biofvm {
 
 class microenvironment_builder
 {
-	std::string name, time_units, space_units;
+	// everything known about one substrate, including its six boundary faces (index [is_max][axis])
+	struct substrate_spec
+	{
+		std::string name, units;
+		real_t diffusion_coefficient = 0, decay_rate = 0, initial_condition = 0;
+		std::array<std::array<real_t, 3>, 2> face_value {};
+		std::array<std::array<bool, 3>, 2> face_condition {};
+	};
 
+	// one interior Dirichlet voxel with a value and a condition per substrate
+	struct dirichlet_node
+	{
+		point_t<index_t, 3> voxel;
+		std::vector<real_t> values;
+		std::vector<bool> conditions;
+	};
+
+	std::string name, time_units, space_units;
 	real_t time_step = 0.01;
 	std::optional<cartesian_mesh> mesh_;
 
-	std::vector<std::string> substrates_names;
-	std::vector<std::string> substrates_units;
-
-	std::vector<real_t> diffusion_coefficients;
-	std::vector<real_t> decay_rates;
-	std::vector<real_t> initial_conditions;
-
-	std::vector<index_t> dirichlet_voxels;
-	std::vector<real_t> dirichlet_values;
-	std::vector<bool> dirichlet_conditions;
-
-	std::vector<point_t<real_t, 3>> boundary_dirichlet_mins_values;
-	std::vector<point_t<real_t, 3>> boundary_dirichlet_maxs_values;
-	std::vector<point_t<bool, 3>> boundary_dirichlet_mins_conditions;
-	std::vector<point_t<bool, 3>> boundary_dirichlet_maxs_conditions;
+	std::vector<substrate_spec> substrates_;
+	std::vector<dirichlet_node> nodes_;
 
 	bulk_func_t supply_rate_func, uptake_rate_func, supply_target_densities_func;
 

@@ -74,6 +74,16 @@ static void require_init(const bfvm_ctx* ctx)
 		throw std::logic_error("solver not initialized (call bfvm_initialize first)");
 }
 
+// Every context entry point that touches device data runs this first: work of a batch holding the context
+// (on the batch's own stream) completes before the context reads or rewrites its buffers, and the next batch
+// step waits for whatever this call enqueues on the context's stream.
+static void enter_ctx(bfvm_ctx* ctx)
+{
+	for (bfvm_batch* b : ctx->batches)
+		BFVM_HIP(hipStreamSynchronize(b->stream));
+	ctx->stream_used = true;
+}
+
 static void free_agents(bfvm_ctx* ctx)
 {
 	auto& a = ctx->ag;
@@ -251,6 +261,8 @@ int bfvm_destroy(bfvm_ctx* ctx)
 	if (!ctx)
 		return BFVM_OK;
 	return guarded("bfvm_destroy", [&] {
+		if (!ctx->batches.empty())
+			throw std::logic_error("the context belongs to a batch: destroy the batch first");
 		BFVM_HIP(hipSetDevice(ctx->device));
 		BFVM_HIP(hipStreamSynchronize(ctx->stream));
 		free_graphs(ctx);
@@ -268,6 +280,7 @@ int bfvm_initialize(bfvm_ctx* ctx, const bfvm_microenvironment_desc* desc)
 		if (!ctx || !desc)
 			throw std::invalid_argument("null argument");
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		BFVM_HIP(hipStreamSynchronize(ctx->stream));
 		const int dims = desc->dims;
 		const int S = desc->substrates_count;
@@ -463,6 +476,7 @@ int bfvm_bulk_store_rates(bfvm_ctx* ctx, const float* supply_rates, const float*
 		if (!supply_rates || !uptake_rates || !supply_target_densities)
 			throw std::invalid_argument("null rate field");
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		BFVM_HIP(hipStreamSynchronize(ctx->stream));
 		const size_t n = ctx->mesh.V * ctx->mesh.S;
 		const float dt = ctx->mesh.dt;
@@ -491,6 +505,7 @@ int bfvm_bulk_solve(bfvm_ctx* ctx)
 		if (!ctx->bulk_num)
 			throw std::logic_error("no bulk rates uploaded (call bfvm_bulk_store_rates first)");
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		launch_bulk(ctx);
 		return BFVM_OK;
 	});
@@ -522,6 +537,7 @@ int bfvm_set_sweep_algorithm(bfvm_ctx* ctx, int32_t algorithm, int32_t chunk)
 		if (chunk < 0 || chunk > 64)
 			throw std::invalid_argument("chunk must be in [0, 64]");
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		ctx->algorithm = algorithm;
 		for (int a = 0; a < 3; a++)
 			ctx->chunk_request[a] = chunk;
@@ -546,6 +562,7 @@ int bfvm_set_sweep_tuning(bfvm_ctx* ctx, int32_t axis, int32_t chunk, int32_t li
 		if (chunk < 0 || chunk > 64 || lines_per_group < 0 || lines_per_group > 1024)
 			throw std::invalid_argument("chunk must be in [0, 64] and lines_per_group in [0, 1024]");
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		const int old_chunk = ctx->chunk_request[axis], old_lines = ctx->lines_request[axis];
 		ctx->chunk_request[axis] = chunk;
 		ctx->lines_request[axis] = lines_per_group;
@@ -568,6 +585,19 @@ int bfvm_set_sweep_tuning(bfvm_ctx* ctx, int32_t axis, int32_t chunk, int32_t li
 			launch_partition_coefficients(ctx, axis);
 			BFVM_HIP(hipStreamSynchronize(ctx->stream));
 		}
+		ctx->epoch++;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_set_sweep_workgroups(bfvm_ctx* ctx, int32_t workgroups)
+{
+	return guarded("bfvm_set_sweep_workgroups", [&] {
+		if (!ctx)
+			throw std::invalid_argument("null context");
+		if (workgroups < 0)
+			throw std::invalid_argument("workgroups must be non-negative");
+		ctx->sweep_workgroups = workgroups;
 		ctx->epoch++;
 		return BFVM_OK;
 	});
@@ -630,6 +660,7 @@ int bfvm_store_densities(bfvm_ctx* ctx, const float* densities)
 		if (!densities)
 			throw std::invalid_argument("null densities");
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		const size_t n = ctx->mesh.V * ctx->mesh.S;
 		if (ctx->storage == BFVM_STORAGE_F32)
 		{
@@ -667,6 +698,7 @@ int bfvm_store_densities_device(bfvm_ctx* ctx, const float* device_densities)
 		if (!device_densities)
 			throw std::invalid_argument("null densities");
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		const size_t n = ctx->mesh.V * ctx->mesh.S;
 		if (ctx->storage == BFVM_STORAGE_F32)
 		{
@@ -700,6 +732,7 @@ int bfvm_store_agents(bfvm_ctx* ctx, const bfvm_agents_desc* agents)
 		if (!agents || agents->agents_count < 0)
 			throw std::invalid_argument("bad agents description");
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		BFVM_HIP(hipStreamSynchronize(ctx->stream));
 		const size_t A = (size_t)agents->agents_count;
 		const size_t S = ctx->mesh.S;
@@ -771,6 +804,7 @@ int bfvm_load_densities(bfvm_ctx* ctx, float* densities)
 		if (!densities)
 			throw std::invalid_argument("null densities");
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		const size_t n = ctx->mesh.V * ctx->mesh.S;
 		if (ctx->storage == BFVM_STORAGE_F32)
 		{
@@ -798,6 +832,7 @@ int bfvm_load_gradients(bfvm_ctx* ctx, float* gradients)
 		if (!gradients)
 			throw std::invalid_argument("null gradients");
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		BFVM_HIP(hipMemcpyAsync(gradients, ctx->grad, sizeof(float) * ctx->grad_elems, hipMemcpyDeviceToHost,
 								ctx->stream));
 		BFVM_HIP(hipStreamSynchronize(ctx->stream));
@@ -810,6 +845,7 @@ int bfvm_load_internalized(bfvm_ctx* ctx, float* internalized_substrates)
 	return guarded("bfvm_load_internalized", [&] {
 		require_init(ctx);
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		const size_t n = (size_t)ctx->ag.count * ctx->mesh.S;
 		if (n > 0)
 		{
@@ -828,6 +864,7 @@ int bfvm_diffusion_solve(bfvm_ctx* ctx)
 	return guarded("bfvm_diffusion_solve", [&] {
 		require_init(ctx);
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		enqueue_diffusion(ctx);
 		return BFVM_OK;
 	});
@@ -838,6 +875,7 @@ int bfvm_dirichlet_solve(bfvm_ctx* ctx)
 	return guarded("bfvm_dirichlet_solve", [&] {
 		require_init(ctx);
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		enqueue_dirichlet(ctx);
 		return BFVM_OK;
 	});
@@ -849,6 +887,7 @@ int bfvm_gradient_solve(bfvm_ctx* ctx)
 		require_init(ctx);
 		check_gradient(ctx);
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		launch_gradient(ctx);
 		return BFVM_OK;
 	});
@@ -859,6 +898,7 @@ int bfvm_cell_simulate(bfvm_ctx* ctx, int32_t recompute)
 	return guarded("bfvm_cell_simulate", [&] {
 		require_init(ctx);
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		launch_cell(ctx, recompute != 0);
 		return BFVM_OK;
 	});
@@ -873,6 +913,7 @@ int bfvm_cell_release(bfvm_ctx* ctx, int32_t agent_index)
 		if (agent_index < 0 || agent_index >= ctx->ag.count)
 			throw std::invalid_argument("agent index out of range");
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		launch_cell_release(ctx, agent_index);
 		return BFVM_OK;
 	});
@@ -885,6 +926,7 @@ int bfvm_timestep(bfvm_ctx* ctx, int32_t recompute, int32_t with_gradient)
 		if (with_gradient)
 			check_gradient(ctx);
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		auto& slot = ctx->graphs[(recompute ? 2 : 0) + (with_gradient ? 1 : 0)];
 		if (!slot.exec || slot.epoch != ctx->epoch)
 		{
@@ -924,6 +966,7 @@ int bfvm_wait_for_all(bfvm_ctx* ctx)
 		if (!ctx)
 			throw std::invalid_argument("null context");
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		BFVM_HIP(hipStreamSynchronize(ctx->stream));
 		return BFVM_OK;
 	});
@@ -974,6 +1017,7 @@ int bfvm_exact_coefficients(bfvm_ctx* ctx, int32_t axis, float* b, float* c, flo
 		if (axis < 0 || axis >= ctx->mesh.dims)
 			throw std::invalid_argument("axis out of range");
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		BFVM_HIP(hipStreamSynchronize(ctx->stream));
 		const auto& t = ctx->axes[axis];
 		const size_t S = ctx->mesh.S;
@@ -996,6 +1040,7 @@ int bfvm_voxel_positions(bfvm_ctx* ctx, int32_t count, const float* positions, i
 		if (count == 0)
 			return BFVM_OK;
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		float* d_pos = nullptr;
 		int* d_out = nullptr;
 		BFVM_HIP(hipMalloc(&d_pos, sizeof(float) * count * ctx->mesh.dims));
@@ -1036,6 +1081,7 @@ int bfvm_benchmark(bfvm_ctx* ctx, int32_t what, int32_t iterations, float* avg_m
 		if (what == BFVM_KERNEL_GRADIENT || what == BFVM_PHASE_GRADIENT || what == BFVM_PHASE_TIMESTEP)
 			check_gradient(ctx);
 		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
 		auto one = [&]() {
 			switch (what)
 			{
@@ -1140,6 +1186,18 @@ static void check_batch(const bfvm_batch* b)
 								   " was re-initialized after bfvm_batch_create; create the batch again");
 }
 
+// A batch step reads every instance's buffers on the batch's stream: first let the work that context calls
+// enqueued on the instances' own streams finish.
+static void join_instance_streams(bfvm_batch* b)
+{
+	for (bfvm_ctx* c : b->ctx)
+		if (c->stream_used)
+		{
+			BFVM_HIP(hipStreamSynchronize(c->stream));
+			c->stream_used = false;
+		}
+}
+
 static uint64_t batch_epoch(const bfvm_batch* b)
 {
 	uint64_t e = b->epoch;
@@ -1242,7 +1300,10 @@ int bfvm_batch_create(bfvm_ctx* const* contexts, int32_t count, bfvm_batch** out
 			b->storage = contexts[0]->storage;
 			BFVM_HIP(hipSetDevice(b->device));
 			for (bfvm_ctx* c : b->ctx)
+			{
 				BFVM_HIP(hipStreamSynchronize(c->stream));
+				c->stream_used = false;
+			}
 			BFVM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
 			BFVM_HIP(hipMalloc(&b->rounding, sizeof(unsigned)));
 			BFVM_HIP(hipMemsetAsync(b->rounding, 0, sizeof(unsigned), b->stream));
@@ -1257,6 +1318,8 @@ int bfvm_batch_create(bfvm_ctx* const* contexts, int32_t count, bfvm_batch** out
 			delete b;
 			throw;
 		}
+		for (bfvm_ctx* c : b->ctx)
+			c->batches.push_back(b);
 		*out = b;
 		return BFVM_OK;
 	});
@@ -1270,6 +1333,8 @@ int bfvm_batch_destroy(bfvm_batch* batch)
 		BFVM_HIP(hipSetDevice(batch->device));
 		if (batch->stream)
 			BFVM_HIP(hipStreamSynchronize(batch->stream));
+		for (bfvm_ctx* c : batch->ctx)
+			c->batches.erase(std::remove(c->batches.begin(), c->batches.end(), batch), c->batches.end());
 		free_batch(batch);
 		delete batch;
 		return BFVM_OK;
@@ -1284,6 +1349,7 @@ int bfvm_batch_timestep(bfvm_batch* batch, int32_t recompute, int32_t with_gradi
 			for (const bfvm_ctx* c : batch->ctx)
 				check_gradient(c);
 		BFVM_HIP(hipSetDevice(batch->device));
+		join_instance_streams(batch);
 		auto& slot = batch->graphs[(recompute ? 2 : 0) + (with_gradient ? 1 : 0)];
 		const uint64_t epoch = batch_epoch(batch);
 		refresh_batch_cell(batch, epoch);
@@ -1337,6 +1403,7 @@ int bfvm_batch_benchmark(bfvm_batch* batch, int32_t what, int32_t iterations, fl
 			for (const bfvm_ctx* c : batch->ctx)
 				check_gradient(c);
 		BFVM_HIP(hipSetDevice(batch->device));
+		join_instance_streams(batch);
 		refresh_batch_cell(batch, batch_epoch(batch));
 		auto one = [&]() {
 			switch (what)

@@ -357,6 +357,8 @@ struct graph_slot
 
 } // namespace bfvm
 
+struct bfvm_batch;
+
 struct bfvm_ctx
 {
 	int device = 0;
@@ -400,6 +402,7 @@ struct bfvm_ctx
 	int chunk_request[3] = { 0, 0, 0 };
 	int lines_request[3] = { 0, 0, 0 };
 	int gradient_groups = 0; // workgroups the 3-D gradient aims for (0 = automatic)
+	int sweep_workgroups = 0; // most workgroups of a partitioned sweep launch (0 = the built-in cap)
 
 	bfvm::agents_buffers ag;
 	int reduction = BFVM_REDUCTION_ATOMIC; // combining of same-voxel agents, applied at the next agents upload
@@ -409,6 +412,11 @@ struct bfvm_ctx
 	uint64_t epoch = 1;		 // bumps whenever captured kernel arguments would change
 	uint64_t generation = 0; // bumps at every bfvm_initialize (buffers reallocated)
 	bfvm::graph_slot graphs[4];
+
+	// batches this context belongs to: a context call first waits for their streams, and a batch step first
+	// waits for the context's stream when a context call may have enqueued work on it (stream_used)
+	std::vector<bfvm_batch*> batches;
+	bool stream_used = false;
 };
 
 // Many microenvironments advanced together (bfvm_batch_*): one launch per sweep axis and one for the

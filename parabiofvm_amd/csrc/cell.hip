@@ -550,6 +550,9 @@ void refresh_batch_cell(bfvm_batch* b, uint64_t epoch)
 	auto& cp = b->cell;
 	if (cp.epoch == epoch)
 		return;
+	// the per-instance arguments and work items are rewritten below with blocking copies, which the batch's
+	// non-blocking stream is not ordered against: steps still in flight must finish first
+	BFVM_HIP(hipStreamSynchronize(b->stream));
 	cp.batched = true;
 	for (const bfvm_ctx* c : b->ctx)
 		cp.batched &= !c->ag.sorted;

@@ -1655,7 +1655,8 @@ static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const a
 	const part_args a = make_part_args(g, p, t.coef, t.red, t.A);
 	const int threads = p.threads;
 	const size_t lds = partition_lds_bytes(p, g.S, p.lpg);
-	const int blocks = (int)std::min<int64_t>(a.groups, sweep_max_blocks);
+	const int64_t cap = ctx->sweep_workgroups > 0 ? ctx->sweep_workgroups : sweep_max_blocks;
+	const int blocks = (int)std::min<int64_t>(a.groups, cap);
 	hipLaunchKernelGGL((sweep_partitioned_kernel<M, AXIS, V, K>), dim3(blocks), dim3(threads), lds, ctx->stream, a);
 	BFVM_HIP(hipGetLastError());
 }
@@ -1775,7 +1776,7 @@ static void launch_staged_axis(bfvm_ctx* ctx, const sweep_geometry& g, const axi
 	const size_t lds = tile_lds_bytes(ta, g.S, ctx->storage);
 	if (lds > staged_lds_limit)
 		throw std::runtime_error("staged sweep: tile does not fit LDS");
-	const int blocks = (int)std::min<int64_t>(ta.tiles, 4096);
+	const int blocks = (int)std::min<int64_t>(ta.tiles, ctx->sweep_workgroups > 0 ? ctx->sweep_workgroups : 4096);
 	hipLaunchKernelGGL((sweep_staged_kernel<AXIS, K>), dim3(blocks), dim3(tile_threads(ta)), lds, ctx->stream, ta);
 	BFVM_HIP(hipGetLastError());
 }

@@ -158,6 +158,10 @@ class DeviceSolver:
         check("bfvm_store_densities", self.lib.bfvm_store_densities(self.h, self.densities.ctypes.data))
         self.store_agents()
 
+    def store_densities(self):
+        """Densities only (agents untouched)."""
+        check("bfvm_store_densities", self.lib.bfvm_store_densities(self.h, self.densities.ctypes.data))
+
     def store_densities_device(self, device_ptr):
         """Densities from device memory (a float32 [V*S] buffer on this context's device), enqueued on the
         context's stream."""
@@ -230,6 +234,10 @@ class DeviceSolver:
 
     def set_gradient_tuning(self, workgroups):
         check("bfvm_set_gradient_tuning", self.lib.bfvm_set_gradient_tuning(self.h, int(workgroups)))
+
+    def set_sweep_workgroups(self, workgroups):
+        """Most workgroups a sweep launches (0 = automatic); fewer workgroups run more passes each."""
+        check("bfvm_set_sweep_workgroups", self.lib.bfvm_set_sweep_workgroups(self.h, int(workgroups)))
 
     def set_sweep_tuning(self, axis, chunk, lines_per_group):
         check("bfvm_set_sweep_tuning", self.lib.bfvm_set_sweep_tuning(self.h, int(axis), int(chunk),

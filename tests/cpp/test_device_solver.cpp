@@ -478,6 +478,233 @@ static void bulk_D3(bool from_team)
 				}
 }
 
+// Agent lifecycle across the boundary (include/BioFVM/agent_container_base.h:61-92, src/agent_data.cpp:27-53,
+// CUDA/cell_solver.cpp:180-211): agents one per voxel, a secretion step, a swap-remove from the middle, growth
+// past the device allocation, an agent moved to another voxel, then steps with and without recompute.  Every
+// voxel is checked against the single-agent closed form (host/cell_solver.h:383-401) in double precision:
+// D' = (D + num) / (1 + den) + fac and I' = I - vv ((num - den D) / (1 + den) + fac), with num = S T dt vol / vv,
+// den = (U + S) dt vol / vv, fac = N dt / vv; voxels without an agent keep their density.
+static void agents_lifecycle()
+{
+	cartesian_mesh mesh(3, { 0, 0, 0 }, { 120, 100, 80 }, { 20, 20, 20 }); // 6 x 5 x 4 voxels
+	auto m = default_microenv(mesh);
+	m.diffusion_time_step = 0.01f;
+	m.compute_internalized_substrates = true;
+	const index_t S = m.substrates_count;
+	const double dt = m.diffusion_time_step, vv = 8000;
+	for (std::size_t i = 0; i < m.mesh.voxel_count() * S; i++)
+		m.substrate_densities[i] = 0.5f + 0.001f * (real_t)(i % 97);
+
+	// agent k sits in voxel `cell` (x, y, z); its rates follow from k
+	auto place = [&](agent* a, index_t k, index_t cell) {
+		const index_t x = cell % 6, y = (cell / 6) % 5, z = cell / 30;
+		for (index_t sb = 0; sb < S; sb++)
+		{
+			a->secretion_rates()[sb] = 1.0f + 0.25f * (real_t)k + (real_t)sb;
+			a->uptake_rates()[sb] = 0.5f + 0.125f * (real_t)k;
+			a->saturation_densities()[sb] = 2.0f + 0.5f * (real_t)sb;
+			a->net_export_rates()[sb] = 10.0f * (real_t)(k % 3);
+			a->fraction_released_at_death()[sb] = 0.5f;
+		}
+		a->volume() = 1000 + 10 * (real_t)k;
+		a->position()[0] = 20 * (real_t)x + 7;
+		a->position()[1] = 20 * (real_t)y + 11;
+		a->position()[2] = 20 * (real_t)z + 13;
+	};
+	auto voxel_of = [&](agent* a) {
+		const auto v = a->voxel_index();
+		return ((std::size_t)v[2] * 5 + v[1]) * 6 + v[0];
+	};
+	std::vector<agent*> live;
+	index_t next_k = 0, next_cell = 0;
+	auto make = [&](int count) {
+		for (int i = 0; i < count; i++)
+		{
+			agent* a = m.agents->create_agent();
+			place(a, next_k++, next_cell);
+			next_cell += 7; // distinct voxels: 7 is coprime with the 120 voxels
+			next_cell %= 120;
+			live.push_back(a);
+		}
+	};
+	auto& s = the_solver();
+	// one step and its closed-form check over every voxel
+	auto step_and_check = [&](bool recompute) {
+		std::vector<double> before(m.substrate_densities.get(), m.substrate_densities.get() + m.mesh.voxel_count() * S);
+		std::vector<double> before_I;
+		for (agent* a : live)
+			for (index_t sb = 0; sb < S; sb++)
+				before_I.push_back(a->internalized_substrates()[sb]);
+		s.store_data_to_solver(m);
+		s.cell.simulate_secretion_and_uptake(m, recompute);
+		s.load_data_from_solver(m);
+		std::vector<double> expect = before;
+		for (std::size_t i = 0; i < live.size(); i++)
+		{
+			agent* a = live[i];
+			const std::size_t v = voxel_of(a);
+			for (index_t sb = 0; sb < S; sb++)
+			{
+				const double Sr = a->secretion_rates()[sb], U = a->uptake_rates()[sb], T = a->saturation_densities()[sb];
+				const double N = a->net_export_rates()[sb], vol = a->volume();
+				const double num = Sr * T * dt * vol / vv, den = (U + Sr) * dt * vol / vv, fac = N * dt / vv;
+				const double D = before[v * S + sb];
+				expect[v * S + sb] = (D + num) / (1 + den) + fac;
+				const double I = before_I[i * S + sb] - vv * ((num - den * D) / (1 + den) + fac);
+				EXPECT_FLOAT_EQ(a->internalized_substrates()[sb], I);
+			}
+		}
+		for (std::size_t i = 0; i < expect.size(); i++)
+			EXPECT_FLOAT_EQ(m.substrate_densities[i], expect[i]);
+	};
+
+	make(6);
+	s.initialize(m);
+	step_and_check(true);
+
+	// swap-remove from the middle: the last agent takes slot 2, its handle and data follow
+	agent* last = live.back();
+	const real_t last_secretion = last->secretion_rates()[1];
+	m.agents->remove_at(2);
+	live.erase(live.begin() + 2);
+	live.pop_back();
+	live.insert(live.begin() + 2, last);
+	EXPECT_FLOAT_EQ(m.agents->agents_count(), 5);
+	EXPECT_FLOAT_EQ(last->index(), 2);
+	EXPECT_FLOAT_EQ(m.agents->get_agent_at(2)->secretion_rates()[1], last_secretion);
+	EXPECT_FLOAT_EQ(m.agents->get_agent_at(2) == last ? 1 : 0, 1);
+	step_and_check(true);
+
+	// growth past the device allocation (6 agents) and an agent moved to a voxel no other agent uses
+	make(14);
+	live[0]->position()[0] = 20 * 1 + 3;
+	live[0]->position()[1] = 3;
+	live[0]->position()[2] = 3; // voxel (1, 0, 0): 7k mod 120 for k < 20 never gives 1
+	bool clash = false;
+	for (std::size_t i = 1; i < live.size(); i++)
+		clash |= voxel_of(live[i]) == voxel_of(live[0]);
+	EXPECT_FLOAT_EQ(clash ? 1 : 0, 0);
+	step_and_check(true);
+	// the reduced factors are reused without recompute
+	step_and_check(false);
+
+	// an agent removed by handle (the last slot: no move), then a step
+	agent* gone = live.back();
+	m.agents->remove_agent(gone);
+	live.pop_back();
+	EXPECT_FLOAT_EQ(m.agents->agents_count(), (double)live.size());
+	step_and_check(true);
+
+	// out-of-range slots are rejected
+	bool threw = false;
+	try
+	{
+		m.agents->get_agent_at((index_t)live.size());
+	}
+	catch (const std::out_of_range&)
+	{
+		threw = true;
+	}
+	EXPECT_FLOAT_EQ(threw ? 1 : 0, 1);
+}
+
+// The builder's bookkeeping (host only, no device work): densities by name, faces created only where some
+// substrate has a condition, interior nodes flattened in insertion order with default conditions.
+static void builder_layout()
+{
+	microenvironment_builder b;
+	b.resize(2, { 0, 0, 0 }, { 100, 60, 0 }, { 20, 20, 20 });
+	b.add_density("o2", "mmHg", 100, 0.1f, 38);
+	b.add_density("glc", "mM", 50, 0.2f, 5);
+	b.add_density("lac", "mM", 25, 0.3f, 0);
+	EXPECT_FLOAT_EQ(b.get_density_index("glc"), 1);
+	bool threw = false;
+	try
+	{
+		b.get_density_index("none");
+	}
+	catch (const std::runtime_error&)
+	{
+		threw = true;
+	}
+	EXPECT_FLOAT_EQ(threw ? 1 : 0, 1);
+	b.add_boundary_dirichlet_conditions(1, { 1, 2, 3 }, { 4, 5, 6 }, { true, false, false }, { false, false, false });
+	b.add_dirichlet_node({ 2, 1, 0 }, { 7, 8, 9 });
+	b.add_dirichlet_node({ 4, 2, 0 }, { 10, 11, 12 }, { false, true, false });
+	threw = false;
+	try
+	{
+		b.add_boundary_dirichlet_conditions(3, { 0, 0, 0 }, { 0, 0, 0 });
+	}
+	catch (const std::runtime_error&)
+	{
+		threw = true;
+	}
+	EXPECT_FLOAT_EQ(threw ? 1 : 0, 1);
+	b.do_compute_internalized_substrates();
+	auto m = b.build();
+	EXPECT_FLOAT_EQ(m.substrates_count, 3);
+	EXPECT_FLOAT_EQ(m.mesh.grid_shape[0], 5);
+	EXPECT_FLOAT_EQ(m.mesh.grid_shape[1], 3);
+	EXPECT_FLOAT_EQ(m.diffusion_coefficients[2], 25);
+	EXPECT_FLOAT_EQ(m.decay_rates[1], 0.2f);
+	EXPECT_FLOAT_EQ(m.substrate_densities[0], 38);
+	EXPECT_FLOAT_EQ(m.substrate_densities[1], 5);
+	EXPECT_FLOAT_EQ(m.compute_internalized_substrates ? 1 : 0, 1);
+	// only the x- face has a condition (substrate 1)
+	EXPECT_FLOAT_EQ(m.dirichlet_min_boundary_values[0] ? 1 : 0, 1);
+	EXPECT_FLOAT_EQ(m.dirichlet_min_boundary_values[1] ? 1 : 0, 0);
+	EXPECT_FLOAT_EQ(m.dirichlet_max_boundary_values[0] ? 1 : 0, 0);
+	EXPECT_FLOAT_EQ(m.dirichlet_min_boundary_values[0][1], 1);
+	EXPECT_FLOAT_EQ(m.dirichlet_min_boundary_conditions[0][1] ? 1 : 0, 1);
+	EXPECT_FLOAT_EQ(m.dirichlet_min_boundary_conditions[0][0] ? 1 : 0, 0);
+	EXPECT_FLOAT_EQ(m.dirichlet_interior_voxels_count, 2);
+	const index_t want_vox[4] = { 2, 1, 4, 2 };
+	for (int i = 0; i < 4; i++)
+		EXPECT_FLOAT_EQ(m.dirichlet_interior_voxels[i], want_vox[i]);
+	for (int i = 0; i < 6; i++)
+		EXPECT_FLOAT_EQ(m.dirichlet_interior_values[i], 7 + i);
+	const bool want_cond[6] = { true, true, true, false, true, false };
+	for (int i = 0; i < 6; i++)
+		EXPECT_FLOAT_EQ(m.dirichlet_interior_conditions[i] ? 1 : 0, want_cond[i] ? 1 : 0);
+}
+
+// The container's slot bookkeeping (host only): swap-remove keeps handles and SoA slots in step, a new agent
+// after a removal starts zeroed.
+static void agent_container_slots()
+{
+	auto m = default_microenv(cartesian_mesh(2, { 0, 0, 0 }, { 100, 100, 0 }, { 20, 20, 20 }));
+	std::vector<agent*> made;
+	for (int i = 0; i < 5; i++)
+	{
+		agent* a = m.agents->create_agent();
+		a->secretion_rates()[0] = (real_t)i;
+		a->internalized_substrates()[1] = 100 + (real_t)i;
+		a->volume() = 10 + (real_t)i;
+		a->position()[1] = 1000 + (real_t)i;
+		made.push_back(a);
+	}
+	EXPECT_FLOAT_EQ(made[3]->index(), 3);
+	m.agents->remove_agent(made[1]); // agent 4 moves into slot 1
+	EXPECT_FLOAT_EQ(m.agents->agents_count(), 4);
+	EXPECT_FLOAT_EQ(made[4]->index(), 1);
+	EXPECT_FLOAT_EQ(made[4]->secretion_rates()[0], 4);
+	EXPECT_FLOAT_EQ(made[4]->internalized_substrates()[1], 104);
+	EXPECT_FLOAT_EQ(made[4]->volume(), 14);
+	EXPECT_FLOAT_EQ(made[4]->position()[1], 1004);
+	EXPECT_FLOAT_EQ(m.agents->get_agent_at(1) == made[4] ? 1 : 0, 1);
+	m.agents->remove_at(3); // the last slot: nothing moves
+	EXPECT_FLOAT_EQ(m.agents->agents_count(), 3);
+	EXPECT_FLOAT_EQ(made[2]->index(), 2);
+	agent* fresh = m.agents->create_agent();
+	EXPECT_FLOAT_EQ(fresh->index(), 3);
+	EXPECT_FLOAT_EQ(fresh->secretion_rates()[0], 0);
+	EXPECT_FLOAT_EQ(fresh->internalized_substrates()[1], 0);
+	EXPECT_FLOAT_EQ(fresh->volume(), 0);
+	EXPECT_FLOAT_EQ(fresh->position()[1], 0);
+	EXPECT_FLOAT_EQ(m.agents->agents_count(), 4);
+}
+
 struct test_case
 {
 	const char* name;
@@ -513,6 +740,9 @@ int main(int argc, char** argv)
 		{ "builder_and_timestep", builder_and_timestep },
 		{ "bulk_D3", [] { bulk_D3(false); } },
 		{ "bulk_D3_from_omp_team", [] { bulk_D3(true); } },
+		{ "agents_lifecycle", agents_lifecycle },
+		{ "host_builder_layout", builder_layout },
+		{ "host_agent_container_slots", agent_container_slots },
 	};
 	for (index_t dims : { 1, 2, 3 })
 		for (bool ci : { true, false })

@@ -200,3 +200,53 @@ def test_batch_vector_widths_with_dirichlet_16bit(storage, pkg, oracle_mod):
         rel = np.linalg.norm(d) / max(np.linalg.norm(o.densities.astype(np.float64)), 1e-30)
         assert rel <= bound, f"instance {k} grid {cfg['grid']} S {cfg['S']}: rel L2 {rel:.2e}"
     batch.close()
+
+
+def test_batch_agents_change_between_steps(pkg, oracle_mod):
+    """Steps of a batch, then, on instance 1, fewer agents and then more agents than its device allocation
+    holds (the batch's secretion plan is rebuilt while steps may still be in flight on the batch's stream, and
+    the context calls must wait for them), then more steps: every instance against the oracle stepping it
+    alone."""
+    from parabiofvm_amd.configs import apply_config, make_batch_config, make_config
+
+    cfgs = make_batch_config(seed=29, count=4, edge=(12, 30), substrates=(2, 8), agent_fraction=0.03)
+    oracles, solvers = [], []
+    for cfg in cfgs:
+        o = oracle_mod.Oracle(*args_of(cfg), dtype=np.float32)
+        g = pkg.DeviceSolver(*args_of(cfg))
+        for be in (o, g):
+            apply_config(cfg, be)
+            be.initialize()
+        g.store()
+        oracles.append(o)
+        solvers.append(g)
+    batch = pkg.Batch(solvers)
+    step = 0
+
+    def steps(n, recompute_first=False):
+        nonlocal step
+        for i in range(n):
+            rc = step % 3 == 0 or (recompute_first and i == 0)  # new agents need their terms computed
+            batch.timestep(rc, True)
+            for o in oracles:
+                o.timestep(rc, True)
+            step += 1
+
+    steps(4)  # no wait: the agent changes below are issued while steps may still run
+    A = cfgs[1]["agents"]["volumes"].size
+    for count in (A // 2, 3 * A + 17):
+        grown = make_config(0, seed=900 + count, shape=cfgs[1]["grid"], substrates=cfgs[1]["S"], agents=count)
+        agents = {k: v.copy() for k, v in grown["agents"].items()}
+        cfgs[1]["agents"] = agents
+        for be in (oracles[1], solvers[1]):
+            be.set_agents({k: v.copy() for k, v in agents.items()})
+        solvers[1].store_agents()
+        oracles[1].initialize()  # the oracle's ballots and reduced arrays follow the agent count
+        steps(3, recompute_first=True)
+    batch.wait()
+    for g in solvers:
+        g.load()
+    for k, (cfg, o, g) in enumerate(zip(cfgs, oracles, solvers)):
+        tag = f"instance {k} grid {cfg['grid']} S {cfg['S']}"
+        assert_close(g.densities, o.densities, f"{tag} densities")
+    batch.close()

@@ -8,7 +8,9 @@ or a transposed operand in the oracle's restatement fails at least one of them:
     operator sum to 1 + decay*dt/dims, so sum(after) = sum(before) / (1 + decay*dt/dims)^dims;
   * the voxel index equals float32 truncation of (p - min)/dx at every voxel boundary and +-1 ulp;
   * the gradient equals numpy central/one-sided differences;
-  * single-agent secretion equals the closed form of host/cell_solver.h:383-401 evaluated in FP64.
+  * single-agent secretion equals the closed form of host/cell_solver.h:383-401 evaluated in FP64;
+  * release at death equals D + I*f/vv at the agent's voxel with I cleared (host/cell_solver.cpp:301-346);
+  * the bulk update equals its geometric closed forms (host/bulk_solver.cpp:23-35).
 """
 import numpy as np
 import pytest
@@ -187,3 +189,60 @@ def test_bulk_closed_forms(oracle_mod):
     d0 = o.densities.copy()
     o.bulk(S, U, T)
     np.testing.assert_allclose(o.densities * (1 + dt * (U + S)) - dt * S * T, d0, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("dims,grid", [(3, (6, 5, 4)), (2, (7, 6, 1)), (1, (15, 1, 1))])
+def test_cell_release_closed_form(dims, grid, oracle_mod):
+    """host/cell_solver.cpp:301-346, release at death, pinned in FP64 by its closed form: at the agent's voxel
+    D' = D + I*f/vv for every substrate and I' = 0, nothing else changes; two agents sharing a voxel add their
+    terms in turn; an agent outside the mesh changes nothing; with internalization off release is a no-op."""
+    S = 3
+    o, _, _, vs = make(oracle_mod, dims, grid, S, np.float64, seed=23)
+    rng = np.random.Generator(np.random.PCG64(9))
+    V = int(np.prod(grid))
+    A = 12
+    vox = rng.choice(V, size=A - 2, replace=False)
+    vox = np.concatenate([vox, [vox[0]]])  # agent A-2 shares agent 0's voxel
+    coords = np.stack([vox % grid[0], (vox // grid[0]) % grid[1], vox // (grid[0] * grid[1])], axis=1)
+    pos = np.zeros((A, dims))
+    for d in range(dims):
+        pos[: A - 1, d] = (coords[:, d] + rng.uniform(0.05, 0.95, size=A - 1)) * vs[d]
+    pos[A - 1, 0] = (grid[0] + 3) * vs[0]  # outside the mesh along x
+    a = {k: rng.uniform(0, 1, size=A * S) for k in ("secretion_rates", "uptake_rates", "saturation_densities",
+                                                     "net_export_rates", "fraction_released_at_death")}
+    a["internalized_substrates"] = rng.uniform(-5e3, 5e4, size=A * S)
+    a["volumes"] = rng.uniform(500, 3000, size=A)
+    a["positions"] = pos.reshape(-1)
+    vv = float(vs[0] * vs[1] * vs[2])  # the reference's integer voxel volume (mesh.cpp:26-32)
+
+    conv = o.set_agents(a)
+    o.set_compute_internalized(True)
+    o.initialize()
+    I0 = conv["internalized_substrates"].copy()
+    expect = o.densities.copy()
+    for i in (3, 0, A - 2):
+        before_I = o.internalized.copy()
+        o.release(i)
+        for s in range(S):
+            expect[vox[i] * S + s] = expect[vox[i] * S + s] + I0[i * S + s] * a["fraction_released_at_death"][i * S + s] / vv
+        np.testing.assert_allclose(o.densities, expect, rtol=1e-15, atol=0)
+        assert np.all(o.internalized[i * S:(i + 1) * S] == 0)
+        keep = np.ones(A * S, dtype=bool)
+        keep[i * S:(i + 1) * S] = False
+        np.testing.assert_array_equal(o.internalized[keep], before_I[keep])
+    # a released agent releases nothing a second time
+    o.release(3)
+    np.testing.assert_allclose(o.densities, expect, rtol=1e-15, atol=0)
+    # outside the mesh: ignored
+    before = o.densities.copy()
+    o.release(A - 1)
+    np.testing.assert_array_equal(o.densities, before)
+    # an index past the agent count is an error
+    with pytest.raises(Exception):
+        o.release(A)
+    # internalization off: no-op on densities and internalized amounts
+    o.set_compute_internalized(False)
+    before_I = o.internalized.copy()
+    o.release(5)
+    np.testing.assert_array_equal(o.densities, before)
+    np.testing.assert_array_equal(o.internalized, before_I)

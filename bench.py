@@ -9,8 +9,10 @@ from a fixed seed.  Each step is one replay of the captured HIP graph; inputs ar
 timed region starts.
 
 Prints ONE JSON line: throughput in voxel*substrate updates/s, the roofline of the dominant kernel (HIP
-events on the solver's stream) and the CPU baseline (the oracle, a restatement of the reference's host
-OpenMP path, timed on this machine's cores on a bounded sample of the same workload).
+events on the solver's stream), the CPU baseline (the oracle, a restatement of the reference's host
+OpenMP path, timed on this machine's cores on a bounded sample of the same workload) and, under "batched",
+the other half of the metric: 64 uneven microenvironments advanced as one batch in FP32 and in 16-bit (FP16)
+storage, with the throughput ratio.
 """
 import argparse
 import json
@@ -42,7 +44,10 @@ def algorithmic_bytes(cfg, e=4):
     face_elems = 2 * (X * Y + Y * Z + X * Z) * S if cfg["faces"] else 0
     per = {
         "sweep_x": 2 * e * VS, "sweep_y": 2 * e * VS, "sweep_z": 2 * e * VS,  # read + write every density once
-        "gradient": (e + 12) * VS,  # read densities, write 3 FP32 components
+        # the step model charges the gradient its 3 FP32 component writes only (in the fused design its read
+        # of the densities belongs to the z sweep's epilogue); gradient_unfused adds that read
+        "gradient": 12 * VS,
+        "gradient_unfused": (e + 12) * VS,
         "dirichlet_interior": n_int * (8 + (5 + e) * S),  # voxel index, values, conditions, density write
         "dirichlet_faces": face_elems * e,
         # positions + volume, then per substrate: reduced sums (12 B), internalized read/write (8 B),
@@ -101,10 +106,11 @@ def cpu_baseline(cfg, budget_s, max_steps):
     }
 
 
-def profiled_traffic(kernel, storage):
-    """HBM bytes per launch of `kernel` on the default 400^3 x 8 workload, from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (calibrated by the copy kernel); None when there is no such summary."""
-    path = os.path.join(ROOT, "profiles", f"traffic_timestep_400cube_8sub_{storage}.md")
+def profiled_traffic(kernel, storage, prefix="traffic_timestep_400cube_8sub"):
+    """HBM bytes per launch of `kernel` on the default workload (400^3 x 8, or the 64-instance batch), from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated by the copy kernel); None when there is no such
+    summary."""
+    path = os.path.join(ROOT, "profiles", f"{prefix}_{storage}.md")
     try:
         text = open(path).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
@@ -113,14 +119,15 @@ def profiled_traffic(kernel, storage):
         return None, None
 
 
-def bench_batch(args):
+def run_batch(args, storage, cfgs=None):
     """64 independent microenvironments of uneven shapes and substrate counts advanced as one batch (one launch
-    per sweep axis and one gradient launch over all instances, the step replayed as one HIP graph)."""
+    per sweep axis and one gradient launch over all instances, the step replayed as one HIP graph).  Returns
+    the batch's figures as a dict."""
     import parabiofvm_amd
     from parabiofvm_amd.configs import apply_config, make_batch_config
 
     t = time.perf_counter()
-    cfgs = make_batch_config(count=args.batch_count)
+    cfgs = cfgs if cfgs is not None else make_batch_config(count=args.batch_count)
     VS = sum(c["updates_per_step"] for c in cfgs)
     A = sum(int(c["agents"]["volumes"].shape[0]) for c in cfgs)
     log(f"[bench] batch of {len(cfgs)} generated in {time.perf_counter() - t:.1f}s: {VS / 1e6:.1f}M voxel*substrates, "
@@ -130,11 +137,15 @@ def bench_batch(args):
         g = parabiofvm_amd.DeviceSolver(c["dims"], c["mins"], c["maxs"], c["voxel_shape"], c["S"], c["dt"], c["D"],
                                         c["decay"], np.zeros(c["S"], dtype=np.float32), device=0)
         apply_config(c, g)
-        g.set_storage(args.storage)
+        g.set_storage(storage)
         g.initialize()
         g.store()
         solvers.append(g)
     batch = parabiofvm_amd.Batch(solvers)
+    if not args.no_autotune:
+        t = time.perf_counter()
+        batch.autotune()
+        log(f"[bench] {storage} batch autotuned in {time.perf_counter() - t:.1f}s")
     for i in range(args.warmup):
         batch.timestep(i % 10 == 0, True)
     batch.wait()
@@ -145,33 +156,50 @@ def bench_batch(args):
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = VS * args.steps / elapsed
-    log(f"[bench] {args.steps} batch steps in {elapsed:.3f}s -> {ms_per_step:.3f} ms/step, {value:.3e} updates/s")
-    e = 4 if args.storage == "f32" else 2
+    log(f"[bench] {storage} batch: {args.steps} steps in {elapsed:.3f}s -> {ms_per_step:.3f} ms/step, "
+        f"{value:.3e} updates/s")
+    e = 4 if storage == "f32" else 2
     kernel_ms = {}
     for k in ("sweep_x", "sweep_y", "sweep_z", "gradient", "secretion", "secretion_recompute"):
         kernel_ms[k] = batch.benchmark(k, args.kernel_iters)
-        log(f"[bench] batched {k}: {kernel_ms[k]:.4f} ms/launch")
-    per = {"sweep_x": 2 * e * VS, "sweep_y": 2 * e * VS, "sweep_z": 2 * e * VS, "gradient": (e + 12) * VS}
+        log(f"[bench] {storage} batched {k}: {kernel_ms[k]:.4f} ms/launch")
+    # step-model bytes (gradient: its 12 B of FP32 component writes) and, for the gradient, the unfused figure
+    per = {"sweep_x": 2 * e * VS, "sweep_y": 2 * e * VS, "sweep_z": 2 * e * VS, "gradient": 12 * VS}
     dominant = max(per, key=lambda k: kernel_ms[k])
     achieved = per[dominant] / (kernel_ms[dominant] * 1e-3) / 1e9
-    result = {
-        "metric": METRIC, "value": value, "unit": "voxel·substrate updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": (f"batch of {len(cfgs)} microenvironments, edges uniform in [32, 200] per axis, 1..16 "
-                                f"substrates, agents 1% of voxels, {args.storage.upper()} storage, gradient every step, "
-                                f"agent recompute every 10 steps"),
-                   "instances": len(cfgs), "voxel_substrates": VS, "agents": A, "storage": args.storage},
+    traffic, source = profiled_traffic("batched_" + dominant, storage, prefix="traffic_batch64")
+    out = {
+        "storage": storage, "ms_per_step": ms_per_step, "value": value,
+        "workload": (f"batch of {len(cfgs)} microenvironments, edges uniform in [32, 200] per axis, 1..16 substrates, "
+                     f"agents 1% of voxels, {storage.upper()} storage, gradient every step, agent recompute every "
+                     f"10 steps"),
+        "instances": len(cfgs), "voxel_substrates": VS, "agents": A,
         "hbm_frac_step": round((6 * e + 12) * VS / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
+        "kernel_gbs": {k: round(per[k] / (kernel_ms[k] * 1e-3) / 1e9, 1) for k in per},
         "roofline": {"bound": "hbm", "kernel": "batched " + dominant, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "traffic_source": source, "bytes_model": "step model (gradient 12 B/VS)",
                      "algorithmic_bytes_per_launch": per[dominant], "avg_launch_ms": round(kernel_ms[dominant], 4)},
-        "cpu_baseline": None,
     }
     batch.close()
     for g in solvers:
         g.close()
+    return out
+
+
+def bench_batch(args):
+    """--batch: the batch alone, one JSON line in the bench contract's form."""
+    b = run_batch(args, args.storage)
+    result = {
+        "metric": METRIC, "value": b["value"], "unit": "voxel·substrate updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": b["workload"], "instances": b["instances"], "voxel_substrates": b["voxel_substrates"],
+                   "agents": b["agents"], "storage": args.storage},
+        "hbm_frac_step": b["hbm_frac_step"], "kernel_ms": b["kernel_ms"], "roofline": b["roofline"],
+        "cpu_baseline": None,
+    }
     print(json.dumps(result), flush=True)
 
 
@@ -214,7 +242,9 @@ def bench_stress(args):
         g.set_storage(st)
         g.initialize()
         g.store()
-        log(f"[bench] {st} context initialized and stored")
+        if not args.no_autotune:
+            g.autotune()
+        log(f"[bench] {st} context initialized, stored and tuned: partitions {[g.partition(a) for a in range(3)]}")
         return g
 
     lo = make(storage)
@@ -289,6 +319,10 @@ def main():
                     help="density storage in HBM between kernels (arithmetic is FP32 in every form)")
     ap.add_argument("--batch", action="store_true", help="the batched workload: many uneven microenvironments")
     ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--batch-16bit", choices=["f16", "bf16"], default="f16",
+                    help="16-bit storage form of the batch half of the default run")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batch half of the default run")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch shapes")
     ap.add_argument("--stress", action="store_true", help="the 640^3 x 16 substrate, 1e7 agent stress workload")
     args = ap.parse_args()
 
@@ -323,6 +357,10 @@ def main():
            "staged": parabiofvm_amd.BFVM_SWEEP_STAGED}[args.algorithm]
     gpu.set_sweep_algorithm(alg, args.chunk)
     gpu.store()
+    if not args.no_autotune and args.algorithm == "partitioned" and args.chunk == 0:
+        t = time.perf_counter()
+        gpu.autotune()
+        log(f"[bench] autotuned in {time.perf_counter() - t:.1f}s")
     log(f"[bench] initialized; partitions {[gpu.partition(a) for a in range(3)]}")
 
     for i in range(args.warmup):
@@ -357,7 +395,13 @@ def main():
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": source,
+                "bytes_model": "step model: sweeps read + write, gradient its 3 FP32 component writes (12 B/VS)",
                 "algorithmic_bytes_per_launch": per_bytes[dominant], "avg_launch_ms": round(kernel_ms[dominant], 4)}
+    if dominant == "gradient":
+        # the gradient as launched here also reads the densities: its bandwidth on those bytes
+        unfused = per_bytes["gradient_unfused"] / (kernel_ms["gradient"] * 1e-3) / 1e9
+        roofline["achieved_with_density_read"] = round(unfused, 1)
+        roofline["frac_with_density_read"] = round(unfused / HBM_PEAK_GBS, 4)
 
     result = {
         "metric": METRIC, "value": value, "unit": "voxel·substrate updates/s", "n_gpus": 1,
@@ -375,6 +419,28 @@ def main():
         "roofline": roofline,
     }
     gpu.close()
+
+    if not args.no_batch:
+        # the other half of the metric: the 64-instance batch in FP32 and in 16-bit storage (FP16, the form that
+        # meets the 1e-3 error bar; BF16 with --batch-16bit bf16), same steps
+        from parabiofvm_amd.configs import make_batch_config
+
+        cfgs = make_batch_config(count=args.batch_count)
+        b32 = run_batch(args, "f32", cfgs)
+        b16 = run_batch(args, args.batch_16bit, cfgs)
+        result["batched"] = {
+            "workload": b32["workload"].replace("F32 storage", "FP32 vs 16-bit storage"),
+            "instances": b32["instances"], "voxel_substrates": b32["voxel_substrates"],
+            "ms_f32": round(b32["ms_per_step"], 4), "ms_16bit": round(b16["ms_per_step"], 4),
+            "storage_16bit": args.batch_16bit,
+            "value_f32": b32["value"], "value_16bit": b16["value"],
+            "ratio_16bit_over_f32": round(b16["value"] / b32["value"], 4),
+            "kernel_ms_f32": b32["kernel_ms"], "kernel_ms_16bit": b16["kernel_ms"],
+            "hbm_frac_step_f32": b32["hbm_frac_step"], "hbm_frac_step_16bit": b16["hbm_frac_step"],
+            "roofline_f32": b32["roofline"], "roofline_16bit": b16["roofline"],
+        }
+        log(f"[bench] batch: FP32 {b32['ms_per_step']:.3f} ms/step, {args.batch_16bit} {b16['ms_per_step']:.3f} ms/step, "
+            f"ratio {result['batched']['ratio_16bit_over_f32']:.3f}")
 
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")

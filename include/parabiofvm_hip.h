@@ -65,11 +65,14 @@ typedef struct bfvm_ctx bfvm_ctx;
 #define BFVM_ERR_HIP (-3)
 #define BFVM_ERR_UNSUPPORTED (-4)
 
-/* Density storage in HBM between kernels (bfvm_set_storage).  All arithmetic is FP32 in registers; the
- * 16-bit forms round to nearest even at every store.  BF16 stores the value itself; FP16 stores
- * value * 2^k_s with a per-substrate power of two chosen at each densities upload so that the largest of
- * the uploaded densities and the Dirichlet values of substrate s lands in [2^10, 2^11) (64x headroom below
- * FP16's largest finite value before a density would overflow). */
+/* Density storage in HBM between kernels (bfvm_set_storage).  All arithmetic is FP32 in registers; values
+ * that are set (uploads, Dirichlet) are rounded to nearest even, values that accumulate (sweep outputs,
+ * secretion) stochastically.  BF16 stores the value itself; FP16 stores value * 2^k_s with a per-substrate
+ * power of two chosen at each densities upload so that a bound on |density| of substrate s lands in
+ * [2^10, 2^11): the largest of the uploaded densities, the Dirichlet values and the agents' saturation and
+ * bulk target densities (the values the solvers move densities towards).  Net export and release at death
+ * raise the bound as they are enqueued; when bound * 2^k_s reaches 2^14 the stored values and the scale are
+ * divided by a power of two on the device before the next step, so FP16 never overflows. */
 #define BFVM_STORAGE_F32 0
 #define BFVM_STORAGE_BF16 1
 #define BFVM_STORAGE_F16 2
@@ -189,6 +192,17 @@ int bfvm_set_sweep_workgroups(bfvm_ctx* ctx, int32_t workgroups);
 /* Workgroups the 3-D gradient kernel aims for by splitting z into chunks (0 = automatic). */
 int bfvm_set_gradient_tuning(bfvm_ctx* ctx, int32_t workgroups);
 
+/* Autotuner, in place of the reference's fixed device launch shapes (work_block_size = 30,
+ * CUDA/diffusion_solver.cpp:11; 256-thread blocks, CUDA/diffusion_solver.cu:947-987): times candidate shapes
+ * of every sweep axis (register chunk, lines or rows per workgroup, workgroup count) and of the 3-D gradient
+ * (workgroups) on this context's mesh and keeps the fastest, as bfvm_set_sweep_tuning /
+ * bfvm_set_sweep_workgroups / bfvm_set_gradient_tuning would set them.  Results are cached per (axis, mesh
+ * shape, substrate count, storage, algorithm) for the process, so a second context of the same shape tunes
+ * at no cost.  The candidates run on a device copy of the densities: the densities are unchanged and the
+ * gradient buffer is recomputed from them.  Call after bfvm_initialize (and after bfvm_set_storage /
+ * bfvm_set_sweep_algorithm have taken effect); synchronous. */
+int bfvm_autotune(bfvm_ctx* ctx);
+
 /* store_data_to_solver / load_data_from_solver parts.  Host pointers, synchronous w.r.t. the host buffer. */
 int bfvm_store_densities(bfvm_ctx* ctx, const float* densities);
 int bfvm_store_agents(bfvm_ctx* ctx, const bfvm_agents_desc* agents);
@@ -270,6 +284,11 @@ int bfvm_batch_create(bfvm_ctx* const* contexts, int32_t count, bfvm_batch** out
 int bfvm_batch_destroy(bfvm_batch* batch);
 int bfvm_batch_timestep(bfvm_batch* batch, int32_t recompute, int32_t with_gradient);
 int bfvm_batch_wait(bfvm_batch* batch);
+/* The batch's autotuner: per sweep axis and thread layout (4-element vectors, one element per thread) the
+ * threads a line group aims for and the workgroup count of the work-list launch, and for the gradient the
+ * tiles per work item and the workgroup count, each timed on copies of the instances' densities.  The
+ * instances' densities are unchanged and their gradients are recomputed.  Synchronous. */
+int bfvm_batch_autotune(bfvm_batch* batch);
 /* Average ms per launch over all instances: BFVM_KERNEL_SWEEP_X/Y/Z and BFVM_KERNEL_GRADIENT (the batched
  * launch alone), BFVM_PHASE_SECRETION(_RECOMPUTE), _DIFFUSION, _GRADIENT and _TIMESTEP. */
 int bfvm_batch_benchmark(bfvm_batch* batch, int32_t what, int32_t iterations, float* avg_ms);

@@ -111,6 +111,8 @@ PROTOTYPES = {
     "bfvm_set_sweep_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "bfvm_set_gradient_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "bfvm_set_sweep_workgroups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "bfvm_autotune": (ctypes.c_int, [ctypes.c_void_p]),
+    "bfvm_batch_autotune": (ctypes.c_int, [ctypes.c_void_p]),
     "bfvm_exact_coefficients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p]),
     "bfvm_voxel_positions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),

@@ -121,6 +121,7 @@ static void free_fields(bfvm_ctx* ctx)
 	dfree(ctx->dscale);
 	dfree(ctx->dinv);
 	dfree(ctx->maxbits);
+	dfree(ctx->rescale_factor);
 	dfree(ctx->rounding_step);
 	dfree(ctx->bulk_num);
 	dfree(ctx->bulk_den);
@@ -222,6 +223,54 @@ static void ensure_partitions(bfvm_ctx* ctx)
 		build_partition(ctx, axis, ctx->chunk_request[axis], ctx->lines_request[axis]);
 		launch_partition_coefficients(ctx, axis);
 	}
+}
+
+// FP16 storage: when the bound on |density| of a substrate, times its scale, reaches 2^14 (4x below FP16's
+// largest finite value), the stored values and the scale of that substrate shrink by the power of two that
+// puts the bound back in [2^10, 2^11).  Enqueued on `stream` ahead of the work that could grow the values; a
+// no-op for the other storage forms and while the bound stays below the threshold.
+static void ensure_f16_headroom(bfvm_ctx* ctx, hipStream_t stream)
+{
+	if (ctx->storage != BFVM_STORAGE_F16)
+		return;
+	const int S = ctx->mesh.S;
+	std::vector<float> factor(S, 1.f);
+	bool any = false;
+	for (int s = 0; s < S; s++)
+	{
+		const double b = ctx->value_bound[s] * ctx->h_scale[s];
+		if (!(b >= 16384.0) || !std::isfinite(ctx->value_bound[s]))
+			continue;
+		int e;
+		std::frexp(ctx->value_bound[s], &e);
+		const float scale = std::ldexp(1.f, std::min(100, std::max(-100, 11 - e)));
+		if (scale < ctx->h_scale[s])
+		{
+			factor[s] = scale / ctx->h_scale[s];
+			ctx->h_scale[s] = scale;
+			any = true;
+		}
+	}
+	if (!any)
+		return;
+	std::vector<float> inv(S);
+	for (int s = 0; s < S; s++)
+		inv[s] = 1.f / ctx->h_scale[s];
+	BFVM_HIP(hipMemcpyAsync(ctx->rescale_factor, factor.data(), sizeof(float) * S, hipMemcpyHostToDevice, stream));
+	launch_rescale_f16(ctx, ctx->rescale_factor, stream);
+	BFVM_HIP(hipMemcpyAsync(ctx->dscale, ctx->h_scale.data(), sizeof(float) * S, hipMemcpyHostToDevice, stream));
+	BFVM_HIP(hipMemcpyAsync(ctx->dinv, inv.data(), sizeof(float) * S, hipMemcpyHostToDevice, stream));
+	BFVM_HIP(hipStreamSynchronize(stream)); // the host vectors above must outlive the copies
+}
+
+// `calls` secretion calls are about to be enqueued: each may add the agents' net export to a density
+static void account_secretion(bfvm_ctx* ctx, int64_t calls, hipStream_t stream)
+{
+	if (ctx->storage != BFVM_STORAGE_F16 || ctx->ag.count == 0)
+		return;
+	for (int s = 0; s < ctx->mesh.S; s++)
+		ctx->value_bound[s] += (double)calls * ctx->export_step[s];
+	ensure_f16_headroom(ctx, stream);
 }
 
 } // namespace bfvm
@@ -326,6 +375,11 @@ int bfvm_initialize(bfvm_ctx* ctx, const bfvm_microenvironment_desc* desc)
 		dalloc_bytes(ctx->dens, VS * storage_bytes(ctx->storage));
 		BFVM_HIP(hipMemsetAsync(ctx->dens, 0, VS * storage_bytes(ctx->storage), ctx->stream));
 		ctx->h_scale.assign(S, 1.f);
+		ctx->value_bound.assign(S, 0.0);
+		ctx->export_step.assign(S, 0.0);
+		ctx->agent_target.assign(S, 0.f);
+		ctx->bulk_target.assign(S, 0.f);
+		dalloc(ctx->rescale_factor, S);
 		dalloc(ctx->dscale, S);
 		dalloc(ctx->dinv, S);
 		dalloc(ctx->maxbits, S);
@@ -494,6 +548,18 @@ int bfvm_bulk_store_rates(bfvm_ctx* ctx, const float* supply_rates, const float*
 		}
 		BFVM_HIP(hipMemcpy(ctx->bulk_num, num.data(), sizeof(float) * n, hipMemcpyHostToDevice));
 		BFVM_HIP(hipMemcpy(ctx->bulk_den, den.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+		if (ctx->storage == BFVM_STORAGE_F16)
+		{
+			// a bulk solve is a weighted average of the density, 0 and the target density
+			const int S = ctx->mesh.S;
+			std::fill(ctx->bulk_target.begin(), ctx->bulk_target.end(), 0.f);
+			for (size_t i = 0; i < n; i++)
+				if (supply_rates[i] != 0.f && std::isfinite(supply_target_densities[i]))
+					ctx->bulk_target[i % S] = std::max(ctx->bulk_target[i % S], std::fabs(supply_target_densities[i]));
+			for (int s = 0; s < S; s++)
+				ctx->value_bound[s] = std::max(ctx->value_bound[s], (double)ctx->bulk_target[s]);
+			ensure_f16_headroom(ctx, ctx->stream);
+		}
 		return BFVM_OK;
 	});
 }
@@ -597,7 +663,8 @@ int bfvm_set_sweep_workgroups(bfvm_ctx* ctx, int32_t workgroups)
 			throw std::invalid_argument("null context");
 		if (workgroups < 0)
 			throw std::invalid_argument("workgroups must be non-negative");
-		ctx->sweep_workgroups = workgroups;
+		for (int a = 0; a < 3; a++)
+			ctx->sweep_workgroups[a] = workgroups;
 		ctx->epoch++;
 		return BFVM_OK;
 	});
@@ -616,6 +683,44 @@ int bfvm_set_gradient_tuning(bfvm_ctx* ctx, int32_t workgroups)
 	});
 }
 
+int bfvm_autotune(bfvm_ctx* ctx)
+{
+	return guarded("bfvm_autotune", [&] {
+		require_init(ctx);
+		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		bool gradient = ctx->mesh.dims == 3;
+		for (int d = 0; d < ctx->mesh.dims; d++)
+			gradient &= ctx->mesh.grid[d] >= 2;
+		// the candidates run on a copy of the densities, so the context's data stay as they are
+		const size_t bytes = ctx->mesh.V * ctx->mesh.S * storage_bytes(ctx->storage);
+		void* scratch = nullptr;
+		BFVM_HIP(hipMalloc(&scratch, bytes));
+		void* real = ctx->dens;
+		try
+		{
+			BFVM_HIP(hipMemcpyAsync(scratch, real, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+			ctx->dens = scratch;
+			autotune_context(ctx, gradient);
+		}
+		catch (...)
+		{
+			ctx->dens = real;
+			(void)hipStreamSynchronize(ctx->stream);
+			(void)hipFree(scratch);
+			throw;
+		}
+		ctx->dens = real;
+		BFVM_HIP(hipFree(scratch));
+		// the gradient timings wrote the gradients of the copy: recompute them from the context's densities
+		if (gradient)
+			launch_gradient(ctx);
+		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		return BFVM_OK;
+	});
+}
+
 // FP16 scale of substrate s: the power of two 2^k with (largest |value|) * 2^k in [2^10, 2^11); 1 when the
 // substrate is all zeros.  BF16 and FP32 keep 1.
 static void choose_scales(bfvm_ctx* ctx, const std::vector<float>& max_abs)
@@ -624,7 +729,8 @@ static void choose_scales(bfvm_ctx* ctx, const std::vector<float>& max_abs)
 	for (int s = 0; s < S; s++)
 	{
 		float scale = 1.f;
-		const float mx = std::max(max_abs[s], ctx->bound_max[s]);
+		const float mx = std::max({ max_abs[s], ctx->bound_max[s], ctx->agent_target[s], ctx->bulk_target[s] });
+		ctx->value_bound[s] = mx;
 		if (ctx->storage == BFVM_STORAGE_F16 && mx > 0.f && std::isfinite(mx))
 		{
 			int e;
@@ -793,6 +899,54 @@ int bfvm_store_agents(bfvm_ctx* ctx, const bfvm_agents_desc* agents)
 				launch_cell_sort(ctx);
 		}
 		BFVM_HIP(hipStreamSynchronize(ctx->stream));
+		if (ctx->storage == BFVM_STORAGE_F16)
+		{
+			// secretion moves a density towards the agents' saturation densities; net export adds N dt / vv per
+			// agent and call to the agent's voxel, so a call raises the largest density by at most the largest
+			// per-voxel sum of |N| dt / vv (diffusion with decay then never raises a maximum)
+			const double vv = (double)ctx->mesh.vs[0] * ctx->mesh.vs[1] * ctx->mesh.vs[2];
+			std::fill(ctx->agent_target.begin(), ctx->agent_target.end(), 0.f);
+			std::fill(ctx->export_step.begin(), ctx->export_step.end(), 0.0);
+			std::unordered_map<int64_t, size_t> slot_of; // voxel -> row of `sums`
+			std::vector<double> sums;
+			for (size_t i = 0; i < A; i++)
+			{
+				bool exporting = false;
+				for (size_t s = 0; s < S; s++)
+				{
+					const float t = std::fabs(agents->saturation_densities[i * S + s]);
+					if (std::isfinite(t))
+						ctx->agent_target[s] = std::max(ctx->agent_target[s], t);
+					exporting |= agents->net_export_rates[i * S + s] != 0.f;
+				}
+				if (!exporting)
+					continue;
+				// the voxel index of src/mesh.cpp:47-53; agents outside the mesh are ignored by the solver
+				int64_t v = 0, mul = 1;
+				bool inside = true;
+				for (size_t d = 0; d < dims; d++)
+				{
+					const float q = (agents->positions[i * dims + d] - (float)ctx->mesh.mins[d]) / (float)ctx->mesh.vs[d];
+					if (!(q >= 0.f) || !(q < (float)ctx->mesh.grid[d]))
+						inside = false;
+					else
+						v += mul * (int64_t)q;
+					mul *= ctx->mesh.grid[d];
+				}
+				if (!inside)
+					continue;
+				auto it = slot_of.emplace(v, sums.size() / S).first;
+				if (it->second * S == sums.size())
+					sums.resize(sums.size() + S, 0.0);
+				for (size_t s = 0; s < S; s++)
+					sums[it->second * S + s] += std::fabs((double)agents->net_export_rates[i * S + s]);
+			}
+			for (size_t k = 0; k < sums.size(); k++)
+				ctx->export_step[k % S] = std::max(ctx->export_step[k % S], sums[k] * ctx->mesh.dt / vv);
+			for (size_t s = 0; s < S; s++)
+				ctx->value_bound[s] = std::max(ctx->value_bound[s], (double)ctx->agent_target[s]);
+			ensure_f16_headroom(ctx, ctx->stream);
+		}
 		return BFVM_OK;
 	});
 }
@@ -899,6 +1053,7 @@ int bfvm_cell_simulate(bfvm_ctx* ctx, int32_t recompute)
 		require_init(ctx);
 		BFVM_HIP(hipSetDevice(ctx->device));
 		enter_ctx(ctx);
+		account_secretion(ctx, 1, ctx->stream);
 		launch_cell(ctx, recompute != 0);
 		return BFVM_OK;
 	});
@@ -914,6 +1069,21 @@ int bfvm_cell_release(bfvm_ctx* ctx, int32_t agent_index)
 			throw std::invalid_argument("agent index out of range");
 		BFVM_HIP(hipSetDevice(ctx->device));
 		enter_ctx(ctx);
+		if (ctx->storage == BFVM_STORAGE_F16)
+		{
+			// the release adds I f / vv at the agent's voxel
+			const int S = ctx->mesh.S;
+			std::vector<float> I(S), f(S);
+			BFVM_HIP(hipMemcpyAsync(I.data(), ctx->ag.intern + (size_t)agent_index * S, sizeof(float) * S,
+									hipMemcpyDeviceToHost, ctx->stream));
+			BFVM_HIP(hipMemcpyAsync(f.data(), ctx->ag.frac + (size_t)agent_index * S, sizeof(float) * S,
+									hipMemcpyDeviceToHost, ctx->stream));
+			BFVM_HIP(hipStreamSynchronize(ctx->stream));
+			const double vv = (double)ctx->mesh.vs[0] * ctx->mesh.vs[1] * ctx->mesh.vs[2];
+			for (int s = 0; s < S; s++)
+				ctx->value_bound[s] += std::fabs((double)I[s] * f[s]) / vv;
+			ensure_f16_headroom(ctx, ctx->stream);
+		}
 		launch_cell_release(ctx, agent_index);
 		return BFVM_OK;
 	});
@@ -927,6 +1097,7 @@ int bfvm_timestep(bfvm_ctx* ctx, int32_t recompute, int32_t with_gradient)
 			check_gradient(ctx);
 		BFVM_HIP(hipSetDevice(ctx->device));
 		enter_ctx(ctx);
+		account_secretion(ctx, 1, ctx->stream);
 		auto& slot = ctx->graphs[(recompute ? 2 : 0) + (with_gradient ? 1 : 0)];
 		if (!slot.exec || slot.epoch != ctx->epoch)
 		{
@@ -1082,6 +1253,8 @@ int bfvm_benchmark(bfvm_ctx* ctx, int32_t what, int32_t iterations, float* avg_m
 			check_gradient(ctx);
 		BFVM_HIP(hipSetDevice(ctx->device));
 		enter_ctx(ctx);
+		if (what == BFVM_PHASE_SECRETION || what == BFVM_PHASE_SECRETION_RECOMPUTE)
+			account_secretion(ctx, (int64_t)iterations + 1, ctx->stream);
 		auto one = [&]() {
 			switch (what)
 			{
@@ -1350,6 +1523,8 @@ int bfvm_batch_timestep(bfvm_batch* batch, int32_t recompute, int32_t with_gradi
 				check_gradient(c);
 		BFVM_HIP(hipSetDevice(batch->device));
 		join_instance_streams(batch);
+		for (bfvm_ctx* c : batch->ctx)
+			account_secretion(c, 1, batch->stream);
 		auto& slot = batch->graphs[(recompute ? 2 : 0) + (with_gradient ? 1 : 0)];
 		const uint64_t epoch = batch_epoch(batch);
 		refresh_batch_cell(batch, epoch);
@@ -1382,6 +1557,67 @@ int bfvm_batch_timestep(bfvm_batch* batch, int32_t recompute, int32_t with_gradi
 	});
 }
 
+int bfvm_batch_autotune(bfvm_batch* batch)
+{
+	return guarded("bfvm_batch_autotune", [&] {
+		check_batch(batch);
+		BFVM_HIP(hipSetDevice(batch->device));
+		join_instance_streams(batch);
+		BFVM_HIP(hipStreamSynchronize(batch->stream));
+		// the candidates run on copies of the instances' densities
+		std::vector<void*> real(batch->ctx.size(), nullptr), scratch(batch->ctx.size(), nullptr);
+		auto restore = [&]() {
+			for (size_t i = 0; i < batch->ctx.size(); i++)
+				if (real[i])
+					batch->ctx[i]->dens = real[i];
+			(void)hipStreamSynchronize(batch->stream);
+			for (void* p : scratch)
+				if (p)
+					(void)hipFree(p);
+		};
+		try
+		{
+			for (size_t i = 0; i < batch->ctx.size(); i++)
+			{
+				bfvm_ctx* c = batch->ctx[i];
+				const size_t bytes = c->mesh.V * c->mesh.S * storage_bytes(c->storage);
+				BFVM_HIP(hipMalloc(&scratch[i], bytes));
+				BFVM_HIP(hipMemcpyAsync(scratch[i], c->dens, bytes, hipMemcpyDeviceToDevice, batch->stream));
+				real[i] = c->dens;
+				c->dens = scratch[i];
+			}
+			autotune_batch(batch);
+		}
+		catch (...)
+		{
+			restore();
+			for (int a = 0; a < 3; a++)
+			{
+				free_batch_sweep(batch, a);
+				prepare_batch_sweep(batch, a);
+			}
+			free_batch_gradient(batch);
+			prepare_batch_gradient(batch);
+			throw;
+		}
+		restore();
+		// the plans hold the instances' buffers: rebuild them on the real densities, recompute the gradients the
+		// timings overwrote, and let the secretion plan refresh
+		for (int a = 0; a < 3; a++)
+		{
+			free_batch_sweep(batch, a);
+			prepare_batch_sweep(batch, a);
+		}
+		free_batch_gradient(batch);
+		prepare_batch_gradient(batch);
+		enqueue_batch_gradient(batch);
+		BFVM_HIP(hipStreamSynchronize(batch->stream));
+		batch->epoch++;
+		batch->cell.epoch = ~0ull;
+		return BFVM_OK;
+	});
+}
+
 int bfvm_batch_wait(bfvm_batch* batch)
 {
 	return guarded("bfvm_batch_wait", [&] {
@@ -1404,6 +1640,9 @@ int bfvm_batch_benchmark(bfvm_batch* batch, int32_t what, int32_t iterations, fl
 				check_gradient(c);
 		BFVM_HIP(hipSetDevice(batch->device));
 		join_instance_streams(batch);
+		if (what == BFVM_PHASE_SECRETION || what == BFVM_PHASE_SECRETION_RECOMPUTE)
+			for (bfvm_ctx* c : batch->ctx)
+				account_secretion(c, (int64_t)iterations + 1, batch->stream);
 		refresh_batch_cell(batch, batch_epoch(batch));
 		auto one = [&]() {
 			switch (what)

@@ -1,0 +1,268 @@
+// Launch-shape autotuner.  The reference fixes its device launch shapes (work_block_size = 30 in
+// CUDA/diffusion_solver.cpp:11, 256-thread blocks in CUDA/diffusion_solver.cu:947-987); here the shapes of
+// every sweep axis (register chunk, lines per workgroup, workgroup count) and of the gradient (workgroups it
+// aims for) are picked by timing candidates on the context's own mesh, and cached per (axis, mesh shape,
+// substrate count, storage) for the process, so that contexts of the same shape tune once.  Batches tune their
+// work-list launches (group size and workgroup count per axis and thread layout, gradient tiles per work item
+// and workgroup count) the same way.
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "bfvm_internal.hpp"
+
+namespace bfvm {
+
+namespace {
+
+struct sweep_shape
+{
+	int chunk = 0, lines = 0, workgroups = 0;
+};
+
+using shape_key = std::tuple<int, int, int, int, int, int, int>; // axis (or -1: gradient), X, Y, Z, S, storage,
+																   // algorithm
+
+std::mutex g_cache_mutex;
+std::map<shape_key, sweep_shape> g_sweep_cache;
+std::map<shape_key, int> g_gradient_cache;
+
+shape_key key_of(const bfvm_ctx* ctx, int axis)
+{
+	const auto& m = ctx->mesh;
+	return shape_key { axis, m.grid[0], m.grid[1], m.grid[2], m.S, ctx->storage, ctx->algorithm };
+}
+
+// Fastest of `reps` launches after one untimed launch, each timed alone with HIP events on `stream`.
+template <typename F>
+float best_ms(hipStream_t stream, int reps, F&& launch)
+{
+	launch();
+	hipEvent_t start = nullptr, stop = nullptr;
+	BFVM_HIP(hipEventCreate(&start));
+	hipError_t err = hipEventCreate(&stop);
+	if (err != hipSuccess)
+	{
+		(void)hipEventDestroy(start);
+		BFVM_HIP(err);
+	}
+	float best = 1e30f;
+	try
+	{
+		for (int r = 0; r < reps; r++)
+		{
+			BFVM_HIP(hipEventRecord(start, stream));
+			launch();
+			BFVM_HIP(hipEventRecord(stop, stream));
+			BFVM_HIP(hipEventSynchronize(stop));
+			float ms = 0;
+			BFVM_HIP(hipEventElapsedTime(&ms, start, stop));
+			best = std::min(best, ms);
+		}
+	}
+	catch (...)
+	{
+		(void)hipEventDestroy(start);
+		(void)hipEventDestroy(stop);
+		throw;
+	}
+	BFVM_HIP(hipEventDestroy(start));
+	BFVM_HIP(hipEventDestroy(stop));
+	return best;
+}
+
+void apply_sweep_shape(bfvm_ctx* ctx, int axis, const sweep_shape& s)
+{
+	ctx->chunk_request[axis] = s.chunk;
+	ctx->lines_request[axis] = s.lines;
+	ctx->sweep_workgroups[axis] = s.workgroups;
+	build_partition(ctx, axis, s.chunk, s.lines);
+	launch_partition_coefficients(ctx, axis);
+}
+
+// Candidates: register chunks (16-element blocks hold 4-substrate vectors; 16-bit storage instantiates 16 and
+// 32 only), lines (y, z) or rows (x) per workgroup, then the workgroup count of the best of those.
+sweep_shape search_sweep(bfvm_ctx* ctx, int axis, int reps)
+{
+	std::vector<int> chunks = { 16, 32 };
+	if (ctx->storage == BFVM_STORAGE_F32)
+		chunks.push_back(64);
+	const std::vector<int> lines = axis == 0 ? std::vector<int> { 0, 1, 2, 4 } : std::vector<int> { 0, 4, 8, 16, 32 };
+	sweep_shape best;
+	float best_t = 1e30f;
+	for (int c : chunks)
+		for (int l : lines)
+		{
+			const sweep_shape cand { c, l, 0 };
+			try
+			{
+				apply_sweep_shape(ctx, axis, cand);
+			}
+			catch (const std::runtime_error&)
+			{
+				continue; // does not fit the register or LDS budget
+			}
+			const float t = best_ms(ctx->stream, reps, [&] { launch_sweep(ctx, axis); });
+			if (t < best_t)
+			{
+				best_t = t;
+				best = cand;
+			}
+		}
+	if (best_t >= 1e30f)
+		throw std::runtime_error("autotune: no sweep shape fits axis " + std::to_string(axis));
+	for (int w : { 1024, 2048, 4096, 16384 })
+	{
+		sweep_shape cand = best;
+		cand.workgroups = w;
+		apply_sweep_shape(ctx, axis, cand);
+		const float t = best_ms(ctx->stream, reps, [&] { launch_sweep(ctx, axis); });
+		if (t < best_t * 0.99f)
+		{
+			best_t = t;
+			best = cand;
+		}
+	}
+	return best;
+}
+
+} // namespace
+
+void autotune_context(bfvm_ctx* ctx, bool tune_gradient)
+{
+	const int reps = 3;
+	for (int axis = 0; axis < ctx->mesh.dims; axis++)
+	{
+		if (ctx->mesh.grid[axis] == 1 || ctx->algorithm != BFVM_SWEEP_PARTITIONED)
+			continue;
+		const shape_key key = key_of(ctx, axis);
+		sweep_shape shape;
+		bool cached = false;
+		{
+			std::lock_guard<std::mutex> lock(g_cache_mutex);
+			auto it = g_sweep_cache.find(key);
+			if (it != g_sweep_cache.end())
+			{
+				shape = it->second;
+				cached = true;
+			}
+		}
+		if (!cached)
+			shape = search_sweep(ctx, axis, reps);
+		apply_sweep_shape(ctx, axis, shape);
+		if (!cached)
+		{
+			std::lock_guard<std::mutex> lock(g_cache_mutex);
+			g_sweep_cache[key] = shape;
+		}
+	}
+	if (tune_gradient && ctx->mesh.dims == 3)
+	{
+		const shape_key key = key_of(ctx, -1);
+		int groups = -1;
+		{
+			std::lock_guard<std::mutex> lock(g_cache_mutex);
+			auto it = g_gradient_cache.find(key);
+			if (it != g_gradient_cache.end())
+				groups = it->second;
+		}
+		if (groups < 0)
+		{
+			float best_t = 1e30f;
+			for (int g : { 4096, 8192, 16384, 32768, 65536 })
+			{
+				ctx->gradient_groups = g;
+				const float t = best_ms(ctx->stream, reps, [&] { launch_gradient(ctx); });
+				if (t < best_t)
+				{
+					best_t = t;
+					groups = g;
+				}
+			}
+			std::lock_guard<std::mutex> lock(g_cache_mutex);
+			g_gradient_cache[key] = groups;
+		}
+		ctx->gradient_groups = groups;
+	}
+	BFVM_HIP(hipStreamSynchronize(ctx->stream));
+	ctx->epoch++;
+}
+
+// Batches: per axis and thread layout the target threads of a line group and the workgroup cap of the
+// work-list launch; for the gradient the tiles per work item and the workgroup cap.  Each candidate rebuilds
+// that part of the plan.
+void autotune_batch(bfvm_batch* b)
+{
+	const int reps = 3;
+	for (int axis = 0; axis < 3; axis++)
+	{
+		if (!b->axes[axis].any)
+			continue;
+		for (int w = 0; w < 2; w++)
+		{
+			if (b->axes[axis].parts[w].nitems == 0)
+				continue;
+			const int max_threads = w == 0 ? 512 : batch_max_threads;
+			int best_threads = b->tuning.group_threads[axis][w], best_blocks = b->tuning.max_blocks[axis][w];
+			float best_t = 1e30f;
+			for (int threads : { 128, 256, 512, 1024 })
+			{
+				if (threads > max_threads)
+					continue;
+				for (int blocks : { 1024, 2048, 4096 })
+				{
+					b->tuning.group_threads[axis][w] = threads;
+					b->tuning.max_blocks[axis][w] = blocks;
+					free_batch_sweep(b, axis);
+					try
+					{
+						prepare_batch_sweep(b, axis);
+					}
+					catch (const std::runtime_error&)
+					{
+						continue;
+					}
+					const float t = best_ms(b->stream, reps, [&] { launch_batched_sweep(b, axis); });
+					if (t < best_t)
+					{
+						best_t = t;
+						best_threads = threads;
+						best_blocks = blocks;
+					}
+				}
+			}
+			b->tuning.group_threads[axis][w] = best_threads;
+			b->tuning.max_blocks[axis][w] = best_blocks;
+			free_batch_sweep(b, axis);
+			prepare_batch_sweep(b, axis);
+		}
+	}
+	if (b->grad.nitems[2] > 0)
+	{
+		int best_tiles = b->tuning.gradient_tiles, best_blocks = b->tuning.gradient_blocks;
+		float best_t = 1e30f;
+		for (int tiles : { 1, 2, 4, 8 })
+			for (int blocks : { 2048, 4096, 8192, 16384 })
+			{
+				b->tuning.gradient_tiles = tiles;
+				b->tuning.gradient_blocks = blocks;
+				free_batch_gradient(b);
+				prepare_batch_gradient(b);
+				const float t = best_ms(b->stream, reps, [&] { launch_batched_gradient(b); });
+				if (t < best_t)
+				{
+					best_t = t;
+					best_tiles = tiles;
+					best_blocks = blocks;
+				}
+			}
+		b->tuning.gradient_tiles = best_tiles;
+		b->tuning.gradient_blocks = best_blocks;
+		free_batch_gradient(b);
+		prepare_batch_gradient(b);
+	}
+	BFVM_HIP(hipStreamSynchronize(b->stream));
+	b->epoch++;
+}
+
+} // namespace bfvm

@@ -345,7 +345,6 @@ struct cell_args
 };
 
 constexpr int batch_chunk = 16; // register block of the staged and batched sweeps (one line per thread)
-constexpr int batch_group_threads = 256; // workgroup size the batched sweeps aim for
 constexpr int batch_max_threads = 1024;
 
 struct graph_slot
@@ -378,6 +377,14 @@ struct bfvm_ctx
 	float* dinv = nullptr;					 // [S] 1 / dscale
 	std::vector<float> h_scale;
 	std::vector<float> bound_max; // [S] largest |Dirichlet value| with its condition set
+	// FP16 headroom: value_bound[s] bounds |density| of substrate s (unscaled).  Diffusion with decay keeps
+	// values within the range of the previous values and the Dirichlet values; secretion and bulk move a value
+	// towards 0 and the agents' saturation / bulk target densities; only net export (export_step per secretion
+	// call, all agents' |N| dt / vv summed as if they shared one voxel) and release at death add to it.  When
+	// value_bound * scale nears FP16's range the stored values and the scale are halved as often as needed.
+	std::vector<double> value_bound, export_step;
+	std::vector<float> agent_target, bulk_target; // [S] largest saturation density / bulk target density
+	float* rescale_factor = nullptr;			   // [S] device factors of a pending rescale
 	unsigned* maxbits = nullptr;  // [S] device scratch of the FP16 scale choice
 	unsigned* rounding_step = nullptr; // device counter keying the stochastic rounding of 16-bit stores
 	float* bulk_num = nullptr; // [V*S] dt*S*T of the bulk supply/uptake fields (null until uploaded)
@@ -402,7 +409,7 @@ struct bfvm_ctx
 	int chunk_request[3] = { 0, 0, 0 };
 	int lines_request[3] = { 0, 0, 0 };
 	int gradient_groups = 0; // workgroups the 3-D gradient aims for (0 = automatic)
-	int sweep_workgroups = 0; // most workgroups of a partitioned sweep launch (0 = the built-in cap)
+	int sweep_workgroups[3] = { 0, 0, 0 }; // most workgroups of a sweep launch per axis (0 = the built-in cap)
 
 	bfvm::agents_buffers ag;
 	int reduction = BFVM_REDUCTION_ATOMIC; // combining of same-voxel agents, applied at the next agents upload
@@ -467,6 +474,16 @@ struct bfvm_batch
 		bool batched = false;
 		uint64_t epoch = ~0ull;
 	} cell;
+	// launch shapes of the work-list kernels (bfvm_batch_autotune picks them): per axis and thread layout
+	// (4-element vectors, one element) the threads a line group aims for and the workgroup cap; for the
+	// gradient the tiles per work item and the workgroup cap
+	struct tuning_t
+	{
+		int group_threads[3][2] = { { 256, 256 }, { 256, 256 }, { 256, 256 } };
+		int max_blocks[3][2] = { { 2048, 2048 }, { 2048, 2048 }, { 2048, 2048 } };
+		int gradient_tiles = 4;
+		int gradient_blocks = 8192;
+	} tuning;
 	uint64_t epoch = 0;
 	bfvm::graph_slot graphs[4];
 };
@@ -488,6 +505,7 @@ void launch_copy(bfvm_ctx* ctx);
 void launch_pack(bfvm_ctx* ctx, const float* src, int64_t offset, int64_t n);
 void launch_unpack(bfvm_ctx* ctx, float* dst, int64_t offset, int64_t n);
 void launch_max_abs(bfvm_ctx* ctx, const float* src, int64_t n, unsigned* out);
+void launch_rescale_f16(bfvm_ctx* ctx, const float* factor, hipStream_t stream);
 void launch_advance_rounding(bfvm_ctx* ctx);
 
 void launch_cell(bfvm_ctx* ctx, bool recompute);
@@ -510,5 +528,9 @@ void free_batch_sweep(bfvm_batch* b, int axis);
 void prepare_batch_gradient(bfvm_batch* b);
 void launch_batched_gradient(bfvm_batch* b);
 void free_batch_gradient(bfvm_batch* b);
+
+// autotuner: time candidate launch shapes on the context's mesh / the batch and keep the fastest
+void autotune_context(bfvm_ctx* ctx, bool tune_gradient);
+void autotune_batch(bfvm_batch* b);
 
 } // namespace bfvm

@@ -1655,7 +1655,7 @@ static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const a
 	const part_args a = make_part_args(g, p, t.coef, t.red, t.A);
 	const int threads = p.threads;
 	const size_t lds = partition_lds_bytes(p, g.S, p.lpg);
-	const int64_t cap = ctx->sweep_workgroups > 0 ? ctx->sweep_workgroups : sweep_max_blocks;
+	const int64_t cap = ctx->sweep_workgroups[AXIS] > 0 ? ctx->sweep_workgroups[AXIS] : sweep_max_blocks;
 	const int blocks = (int)std::min<int64_t>(a.groups, cap);
 	hipLaunchKernelGGL((sweep_partitioned_kernel<M, AXIS, V, K>), dim3(blocks), dim3(threads), lds, ctx->stream, a);
 	BFVM_HIP(hipGetLastError());
@@ -1776,7 +1776,8 @@ static void launch_staged_axis(bfvm_ctx* ctx, const sweep_geometry& g, const axi
 	const size_t lds = tile_lds_bytes(ta, g.S, ctx->storage);
 	if (lds > staged_lds_limit)
 		throw std::runtime_error("staged sweep: tile does not fit LDS");
-	const int blocks = (int)std::min<int64_t>(ta.tiles, ctx->sweep_workgroups > 0 ? ctx->sweep_workgroups : 4096);
+	const int blocks =
+		(int)std::min<int64_t>(ta.tiles, ctx->sweep_workgroups[AXIS] > 0 ? ctx->sweep_workgroups[AXIS] : 4096);
 	hipLaunchKernelGGL((sweep_staged_kernel<AXIS, K>), dim3(blocks), dim3(tile_threads(ta)), lds, ctx->stream, ta);
 	BFVM_HIP(hipGetLastError());
 }
@@ -1908,8 +1909,8 @@ void prepare_batch_sweep(bfvm_batch* b, int axis)
 									 " does not fit 16-element register blocks (axis too long)");
 		p.vec = V;
 		const int SG = axis == 0 ? S / V : 1;
-		const int request = axis == 0 ? std::max(1, batch_group_threads / (SG * p.P))
-									  : std::max(8, batch_group_threads / p.P / 8 * 8);
+		const int target = b->tuning.group_threads[axis][w];
+		const int request = axis == 0 ? std::max(1, target / (SG * p.P)) : std::max(8, target / p.P / 8 * 8);
 		group_shape(axis, SG, p.P, request, max_threads, p.lpg, p.threads);
 		if (p.threads > max_threads)
 			throw std::runtime_error("batched sweep: instance " + std::to_string(i) +
@@ -1944,7 +1945,7 @@ void prepare_batch_sweep(bfvm_batch* b, int axis)
 		BFVM_HIP(hipMalloc(&part.items, sizeof(batch_item) * items[w].size()));
 		BFVM_HIP(hipMemcpy(part.items, items[w].data(), sizeof(batch_item) * items[w].size(), hipMemcpyHostToDevice));
 		part.nitems = (int)items[w].size();
-		part.blocks = std::min(part.nitems, 2048);
+		part.blocks = std::min(part.nitems, b->tuning.max_blocks[axis][w]);
 	}
 }
 

@@ -485,8 +485,8 @@ void free_batch_gradient(bfvm_batch* b)
 	gp = bfvm_batch::gradient_plan {};
 }
 
-// Tiles of 16 z planes; items of 4 tiles (neighbouring rows of one z range, one after the other in one
-// workgroup).
+// Tiles of 16 z planes; items of b->tuning.gradient_tiles tiles (neighbouring rows of one z range, one after
+// the other in one workgroup).
 void prepare_batch_gradient(bfvm_batch* b)
 {
 	auto& gp = b->grad;
@@ -506,9 +506,10 @@ void prepare_batch_gradient(bfvm_batch* b)
 		const int64_t row_tiles = (int64_t)probe.ntiles;
 		args[i] = make_gradient_args(ctx, row_tiles * ((ctx->mesh.grid[2] + 15) / 16), batch_gradient_rows);
 		const int m = args[i].vec == 4 ? 1 : 0;
-		for (int64_t t0 = 0; t0 < args[i].ntiles; t0 += 4)
+		const int per_item = std::max(1, b->tuning.gradient_tiles);
+		for (int64_t t0 = 0; t0 < args[i].ntiles; t0 += per_item)
 		{
-			const batch_item w { (int32_t)i, 0, t0, std::min<int64_t>(args[i].ntiles, t0 + 4) };
+			const batch_item w { (int32_t)i, 0, t0, std::min<int64_t>(args[i].ntiles, t0 + per_item) };
 			items[m].push_back(w);
 			items[2].push_back(w);
 		}
@@ -524,7 +525,7 @@ void prepare_batch_gradient(bfvm_batch* b)
 		BFVM_HIP(hipMalloc(&gp.items[m], sizeof(batch_item) * items[m].size()));
 		BFVM_HIP(hipMemcpy(gp.items[m], items[m].data(), sizeof(batch_item) * items[m].size(), hipMemcpyHostToDevice));
 		gp.nitems[m] = (int)items[m].size();
-		gp.blocks[m] = std::min(gp.nitems[m], 8192);
+		gp.blocks[m] = std::min(gp.nitems[m], b->tuning.gradient_blocks);
 	}
 }
 
@@ -620,6 +621,25 @@ __global__ __launch_bounds__(256) void max_abs_kernel(const float* __restrict__ 
 		if (a > 0.f && a <= 3.4e38f)
 			atomicMax(&out[i % S], __float_as_uint(a));
 	}
+}
+
+// FP16 storage: stored values of substrate s times factor[s] (a power of two, so exact while the result stays
+// normal); grid-stride over the whole densities array
+__global__ __launch_bounds__(256) void rescale_f16_kernel(_Float16* __restrict__ dens, const float* __restrict__ factor,
+														  int S, int64_t n)
+{
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+		dens[i] = (_Float16)((float)dens[i] * factor[i % S]);
+}
+
+void launch_rescale_f16(bfvm_ctx* ctx, const float* factor, hipStream_t stream)
+{
+	const int64_t n = (int64_t)ctx->mesh.V * ctx->mesh.S;
+	const unsigned blocks = (unsigned)std::min<int64_t>(8192, (n + 255) / 256);
+	hipLaunchKernelGGL(rescale_f16_kernel, dim3(blocks), dim3(256), 0, stream, (_Float16*)ctx->dens, factor,
+					   ctx->mesh.S, n);
+	BFVM_HIP(hipGetLastError());
 }
 
 __global__ void advance_rounding_kernel(unsigned* counter)

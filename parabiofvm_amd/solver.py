@@ -235,6 +235,11 @@ class DeviceSolver:
     def set_gradient_tuning(self, workgroups):
         check("bfvm_set_gradient_tuning", self.lib.bfvm_set_gradient_tuning(self.h, int(workgroups)))
 
+    def autotune(self):
+        """Time candidate launch shapes of every sweep axis and the gradient on this mesh and keep the fastest
+        (cached per shape for the process); the densities are unchanged."""
+        check("bfvm_autotune", self.lib.bfvm_autotune(self.h))
+
     def set_sweep_workgroups(self, workgroups):
         """Most workgroups a sweep launches (0 = automatic); fewer workgroups run more passes each."""
         check("bfvm_set_sweep_workgroups", self.lib.bfvm_set_sweep_workgroups(self.h, int(workgroups)))
@@ -298,6 +303,10 @@ class Batch:
 
     def wait(self):
         check("bfvm_batch_wait", self.lib.bfvm_batch_wait(self.h))
+
+    def autotune(self):
+        """Time candidate work-list launch shapes of the batch's sweeps and gradient and keep the fastest."""
+        check("bfvm_batch_autotune", self.lib.bfvm_batch_autotune(self.h))
 
     def benchmark(self, what, iterations):
         ms = ctypes.c_float()

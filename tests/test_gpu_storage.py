@@ -22,10 +22,15 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-# FP32 storage against the FP64 oracle: FP32 coefficient rounding biases every sweep by ~1 ulp the same way, so
-# the error grows about linearly with the step count (~1e-7 per step); the <= 1e-5 per-voxel bar of the FP32 path
-# is against the FP32 oracle (test_gpu_parity).  FP16 meets 1e-3; BF16 is bounded by its quantization floor.
-L2_BOUND = {"f32": 1e-4, "f16": 1e-3, "bf16": 1e-2}
+# The error bar of the 16-bit forms: relative L2 <= 1e-3 against the FP64 oracle.  FP16 storage (scaled) meets
+# it and is the 16-bit form the benchmark reports.  BF16 does not: its 8 significant bits give a quantisation
+# floor of about 2-6e-3 (one rounding of values spread over an octave alone is ~1.7e-3 RMS), so BF16 is checked
+# only against that floor, 1e-2.  FP32 storage against the FP64 oracle: FP32 coefficient rounding biases every
+# sweep by ~1 ulp the same way, so the error grows about linearly with the step count (~1e-7 per step); the
+# <= 1e-5 per-voxel bar of the FP32 path is against the FP32 oracle (test_gpu_parity).
+ERROR_BAR = 1e-3
+BF16_FLOOR = 1e-2
+L2_BOUND = {"f32": 1e-4, "f16": ERROR_BAR, "bf16": BF16_FLOOR}
 
 
 @pytest.fixture(scope="module")
@@ -233,3 +238,59 @@ def test_precision_100cube_200_steps(storage, pkg, oracle_mod):
         with open(os.path.join(out, f"precision_100cube_{storage}.json"), "w") as f:
             json.dump({"workload": "100^3 mesh, 4 substrates, 10k agents, dt 0.01", "rows": rows}, f, indent=1)
     check_rows(rows, storage)
+
+
+def _scales(s, S):
+    scales = np.zeros(S, np.float32)
+    hip = hip_runtime()
+    assert hip.hipMemcpy(scales.ctypes.data, s.device_buffers().density_scales, 4 * S, 2) == 0  # device to host
+    return scales
+
+
+def _secreting_agents(grid, S, A, rng, export):
+    pos = rng.uniform(0, 1, size=(A, 3)) * (np.array(grid) * 20.0)
+    return {"secretion_rates": rng.uniform(0.5, 1.0, size=A * S), "uptake_rates": rng.uniform(0.0, 0.2, size=A * S),
+            "saturation_densities": rng.uniform(50.0, 80.0, size=A * S),
+            "net_export_rates": np.full(A * S, export), "internalized_substrates": np.zeros(A * S),
+            "fraction_released_at_death": np.zeros(A * S), "volumes": np.full(A, 2494.0),
+            "positions": pos.reshape(-1)}
+
+
+@pytest.mark.parametrize("export", [0.0, 2e8])
+def test_f16_headroom_near_zero_field_with_secretion(export, pkg, oracle_mod):
+    """FP16 scale safety: densities uploaded near zero (the upload alone would pick a scale near 2^30) with
+    agents secreting towards saturation densities of 50..80, and, in the second case, a net export that keeps
+    raising the densities past what the first scale could hold.  The scale must come from the agents' targets
+    and shrink as the export accumulates: every value finite and within the error bar of the FP64 oracle."""
+    grid, S, A = (24, 20, 16), 2, 300
+    rng = np.random.Generator(np.random.PCG64(47))
+    D = np.array([2e3, 5e4], np.float32)
+    decay = np.array([0.1, 0.5], np.float32)
+    args = (3, [0, 0, 0], [g * 20 for g in grid], [20, 20, 20], S, 0.01, D, decay, np.zeros(S, np.float32))
+    o = oracle_mod.Oracle(*args, dtype=np.float64)
+    g = pkg.DeviceSolver(*args)
+    g.set_storage("f16")
+    agents = _secreting_agents(grid, S, A, rng, export)
+    init = rng.uniform(0, 1e-6, size=grid[0] * grid[1] * grid[2] * S)
+    for be in (o, g):
+        be.densities[:] = init
+        be.set_agents({k: v.copy() for k, v in agents.items()})
+        be.set_compute_internalized(True)
+        be.initialize()
+    g.store()
+    first = _scales(g, S)
+    assert first.max() <= 2.0 ** 5, first  # from the saturation densities (< 2^7), not from the 1e-6 upload
+    steps = 40
+    for i in range(steps):
+        o.timestep(i % 10 == 0, True)
+        g.timestep(i % 10 == 0, True)
+    g.load()
+    assert np.all(np.isfinite(g.densities))
+    l2, mx = rel_errors(g.densities, o.densities, S)
+    last = _scales(g, S)
+    print(f"export {export:g}: scales {first} -> {last}, max density {np.abs(o.densities).max():.3e}, "
+          f"rel L2 {l2}")
+    assert l2.max() <= ERROR_BAR, l2
+    if export > 0:
+        assert np.abs(o.densities).max() * first.max() > 65504.0  # the first scale would have overflowed
+        assert np.all(last < first), (first, last)

@@ -323,6 +323,9 @@ def main():
                     help="16-bit storage form of the batch half of the default run")
     ap.add_argument("--no-batch", action="store_true", help="skip the batch half of the default run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch shapes")
+    ap.add_argument("--tuning-cache", default=None,
+                    help="launch-shape tuning cache file: read when it exists, written after tuning (a run that "
+                         "finds every shape there times no candidates)")
     ap.add_argument("--stress", action="store_true", help="the 640^3 x 16 substrate, 1e7 agent stress workload")
     args = ap.parse_args()
 
@@ -331,6 +334,21 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path (each GPU box has one MI355X); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.tuning_cache and os.path.exists(args.tuning_cache):
+        from parabiofvm_amd import _native
+
+        _native.tuning_load(args.tuning_cache)
+        log(f"[bench] tuning cache {args.tuning_cache} loaded")
+    try:
+        run(args)
+    finally:
+        if args.tuning_cache and not args.no_autotune:
+            from parabiofvm_amd import _native
+
+            _native.tuning_save(args.tuning_cache)
+
+
+def run(args):
     if args.batch:
         bench_batch(args)
         return

@@ -202,6 +202,11 @@ int bfvm_set_gradient_tuning(bfvm_ctx* ctx, int32_t workgroups);
  * gradient buffer is recomputed from them.  Call after bfvm_initialize (and after bfvm_set_storage /
  * bfvm_set_sweep_algorithm have taken effect); synchronous. */
 int bfvm_autotune(bfvm_ctx* ctx);
+/* The process's tuning cache (bfvm_autotune and bfvm_batch_autotune results) written to / merged from a text
+ * file, so that a later process starts tuned: with every shape of a context or batch in the cache, the
+ * autotune calls apply the cached shapes without timing anything. */
+int bfvm_tuning_save(const char* path);
+int bfvm_tuning_load(const char* path);
 
 /* store_data_to_solver / load_data_from_solver parts.  Host pointers, synchronous w.r.t. the host buffer. */
 int bfvm_store_densities(bfvm_ctx* ctx, const float* densities);

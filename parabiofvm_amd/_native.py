@@ -113,6 +113,8 @@ PROTOTYPES = {
     "bfvm_set_sweep_workgroups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "bfvm_autotune": (ctypes.c_int, [ctypes.c_void_p]),
     "bfvm_batch_autotune": (ctypes.c_int, [ctypes.c_void_p]),
+    "bfvm_tuning_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "bfvm_tuning_load": (ctypes.c_int, [ctypes.c_char_p]),
     "bfvm_exact_coefficients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p]),
     "bfvm_voxel_positions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
@@ -153,3 +155,13 @@ def check(fn, code):
     if code != 0:
         msg = load().bfvm_last_error()
         raise BfvmError(fn, code, msg.decode() if msg else "")
+
+
+def tuning_save(path):
+    """Write the process's launch-shape tuning cache to a text file."""
+    check("bfvm_tuning_save", load().bfvm_tuning_save(os.fsencode(path)))
+
+
+def tuning_load(path):
+    """Merge a tuning cache file written by tuning_save into the process's cache."""
+    check("bfvm_tuning_load", load().bfvm_tuning_load(os.fsencode(path)))

@@ -1564,6 +1564,19 @@ int bfvm_batch_autotune(bfvm_batch* batch)
 		BFVM_HIP(hipSetDevice(batch->device));
 		join_instance_streams(batch);
 		BFVM_HIP(hipStreamSynchronize(batch->stream));
+		if (apply_cached_batch_tuning(batch))
+		{
+			for (int a = 0; a < 3; a++)
+			{
+				free_batch_sweep(batch, a);
+				prepare_batch_sweep(batch, a);
+			}
+			free_batch_gradient(batch);
+			prepare_batch_gradient(batch);
+			BFVM_HIP(hipStreamSynchronize(batch->stream));
+			batch->epoch++;
+			return BFVM_OK;
+		}
 		// the candidates run on copies of the instances' densities
 		std::vector<void*> real(batch->ctx.size(), nullptr), scratch(batch->ctx.size(), nullptr);
 		auto restore = [&]() {
@@ -1614,6 +1627,26 @@ int bfvm_batch_autotune(bfvm_batch* batch)
 		BFVM_HIP(hipStreamSynchronize(batch->stream));
 		batch->epoch++;
 		batch->cell.epoch = ~0ull;
+		return BFVM_OK;
+	});
+}
+
+int bfvm_tuning_save(const char* path)
+{
+	return guarded("bfvm_tuning_save", [&] {
+		if (!path)
+			throw std::invalid_argument("null path");
+		save_tuning(path);
+		return BFVM_OK;
+	});
+}
+
+int bfvm_tuning_load(const char* path)
+{
+	return guarded("bfvm_tuning_load", [&] {
+		if (!path)
+			throw std::invalid_argument("null path");
+		load_tuning(path);
 		return BFVM_OK;
 	});
 }

@@ -4,9 +4,13 @@
 // aims for) are picked by timing candidates on the context's own mesh, and cached per (axis, mesh shape,
 // substrate count, storage) for the process, so that contexts of the same shape tune once.  Batches tune their
 // work-list launches (group size and workgroup count per axis and thread layout, gradient tiles per work item
-// and workgroup count) the same way.
+// and workgroup count) the same way, cached per list of instance shapes.  The cache can be written to and read
+// from a text file, so that a later process starts tuned.
+#include <cstdio>
 #include <map>
 #include <mutex>
+#include <sstream>
+#include <string>
 #include <tuple>
 
 #include "bfvm_internal.hpp"
@@ -26,6 +30,7 @@ using shape_key = std::tuple<int, int, int, int, int, int, int>; // axis (or -1:
 std::mutex g_cache_mutex;
 std::map<shape_key, sweep_shape> g_sweep_cache;
 std::map<shape_key, int> g_gradient_cache;
+std::map<std::string, bfvm_batch::tuning_t> g_batch_cache; // key: storage and every instance's shape
 
 shape_key key_of(const bfvm_ctx* ctx, int axis)
 {
@@ -126,7 +131,119 @@ sweep_shape search_sweep(bfvm_ctx* ctx, int axis, int reps)
 	return best;
 }
 
+std::string batch_key(const bfvm_batch* b)
+{
+	std::ostringstream k;
+	k << b->storage << ':' << b->ctx.size();
+	for (const bfvm_ctx* c : b->ctx)
+		k << ';' << c->mesh.grid[0] << ',' << c->mesh.grid[1] << ',' << c->mesh.grid[2] << ',' << c->mesh.S << ','
+		  << c->mesh.dims;
+	return k.str();
+}
+
 } // namespace
+
+bool apply_cached_batch_tuning(bfvm_batch* b)
+{
+	std::lock_guard<std::mutex> lock(g_cache_mutex);
+	auto it = g_batch_cache.find(batch_key(b));
+	if (it == g_batch_cache.end())
+		return false;
+	b->tuning = it->second;
+	return true;
+}
+
+// Text file, one entry per line:
+//   sweep <axis> <X> <Y> <Z> <S> <storage> <algorithm> <chunk> <lines> <workgroups>
+//   gradient <X> <Y> <Z> <S> <storage> <algorithm> <workgroups>
+//   batch <key> <group threads [3][2]> <max blocks [3][2]> <gradient tiles> <gradient blocks>
+void save_tuning(const char* path)
+{
+	std::lock_guard<std::mutex> lock(g_cache_mutex);
+	FILE* f = std::fopen(path, "w");
+	if (!f)
+		throw std::invalid_argument(std::string("cannot write ") + path);
+	for (const auto& [k, v] : g_sweep_cache)
+		std::fprintf(f, "sweep %d %d %d %d %d %d %d %d %d %d\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
+					 std::get<3>(k), std::get<4>(k), std::get<5>(k), std::get<6>(k), v.chunk, v.lines, v.workgroups);
+	for (const auto& [k, v] : g_gradient_cache)
+		std::fprintf(f, "gradient %d %d %d %d %d %d %d\n", std::get<1>(k), std::get<2>(k), std::get<3>(k),
+					 std::get<4>(k), std::get<5>(k), std::get<6>(k), v);
+	for (const auto& [k, t] : g_batch_cache)
+	{
+		std::fprintf(f, "batch %s", k.c_str());
+		for (int a = 0; a < 3; a++)
+			for (int w = 0; w < 2; w++)
+				std::fprintf(f, " %d", t.group_threads[a][w]);
+		for (int a = 0; a < 3; a++)
+			for (int w = 0; w < 2; w++)
+				std::fprintf(f, " %d", t.max_blocks[a][w]);
+		std::fprintf(f, " %d %d\n", t.gradient_tiles, t.gradient_blocks);
+	}
+	const bool bad = std::ferror(f) != 0;
+	std::fclose(f);
+	if (bad)
+		throw std::runtime_error(std::string("error writing ") + path);
+}
+
+void load_tuning(const char* path)
+{
+	FILE* f = std::fopen(path, "r");
+	if (!f)
+		throw std::invalid_argument(std::string("cannot read ") + path);
+	std::string text;
+	char buf[4096];
+	size_t got;
+	while ((got = std::fread(buf, 1, sizeof buf, f)) > 0)
+		text.append(buf, got);
+	std::fclose(f);
+	std::istringstream lines(text);
+	std::string line;
+	std::lock_guard<std::mutex> lock(g_cache_mutex);
+	int lineno = 0;
+	while (std::getline(lines, line))
+	{
+		lineno++;
+		if (line.empty())
+			continue;
+		std::istringstream in(line);
+		std::string kind;
+		in >> kind;
+		bool ok = false;
+		if (kind == "sweep")
+		{
+			int axis, X, Y, Z, S, st, alg;
+			sweep_shape v;
+			ok = static_cast<bool>(in >> axis >> X >> Y >> Z >> S >> st >> alg >> v.chunk >> v.lines >> v.workgroups);
+			if (ok)
+				g_sweep_cache[shape_key { axis, X, Y, Z, S, st, alg }] = v;
+		}
+		else if (kind == "gradient")
+		{
+			int X, Y, Z, S, st, alg, groups;
+			ok = static_cast<bool>(in >> X >> Y >> Z >> S >> st >> alg >> groups);
+			if (ok)
+				g_gradient_cache[shape_key { -1, X, Y, Z, S, st, alg }] = groups;
+		}
+		else if (kind == "batch")
+		{
+			std::string key;
+			bfvm_batch::tuning_t t;
+			ok = static_cast<bool>(in >> key);
+			for (int a = 0; a < 3 && ok; a++)
+				for (int w = 0; w < 2 && ok; w++)
+					ok = static_cast<bool>(in >> t.group_threads[a][w]);
+			for (int a = 0; a < 3 && ok; a++)
+				for (int w = 0; w < 2 && ok; w++)
+					ok = static_cast<bool>(in >> t.max_blocks[a][w]);
+			ok = ok && static_cast<bool>(in >> t.gradient_tiles >> t.gradient_blocks);
+			if (ok)
+				g_batch_cache[key] = t;
+		}
+		if (!ok)
+			throw std::invalid_argument(std::string(path) + ":" + std::to_string(lineno) + ": not a tuning entry");
+	}
+}
 
 void autotune_context(bfvm_ctx* ctx, bool tune_gradient)
 {
@@ -263,6 +380,8 @@ void autotune_batch(bfvm_batch* b)
 	}
 	BFVM_HIP(hipStreamSynchronize(b->stream));
 	b->epoch++;
+	std::lock_guard<std::mutex> lock(g_cache_mutex);
+	g_batch_cache[batch_key(b)] = b->tuning;
 }
 
 } // namespace bfvm

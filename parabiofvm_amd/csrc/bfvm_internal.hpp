@@ -532,5 +532,8 @@ void free_batch_gradient(bfvm_batch* b);
 // autotuner: time candidate launch shapes on the context's mesh / the batch and keep the fastest
 void autotune_context(bfvm_ctx* ctx, bool tune_gradient);
 void autotune_batch(bfvm_batch* b);
+bool apply_cached_batch_tuning(bfvm_batch* b); // false when this list of instance shapes has no cached tuning
+void save_tuning(const char* path);
+void load_tuning(const char* path);
 
 } // namespace bfvm

@@ -247,6 +247,14 @@ int bfvm_exact_coefficients(bfvm_ctx* ctx, int32_t axis, float* b, float* c, flo
  * kernel the secretion solver uses; out3 gets [count][3] indices, -1 on axes where the point is outside. */
 int bfvm_voxel_positions(bfvm_ctx* ctx, int32_t count, const float* positions, int32_t* out3);
 
+/* bfvm_voxel_positions without any device allocation, for callers that cannot allocate device memory
+ * (code near HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in
+ * the reference.  The positions and indices are staged through a fixed device array of the library (24 MiB per
+ * GPU, held until the process ends) instead of per-call allocations; at most 1,048,576 points per call (more
+ * is BFVM_ERR_ARGUMENT).  Every call on one GPU shares that array: calls that overlap on one GPU (from several
+ * host threads) race.  Synchronous; results equal bfvm_voxel_positions'. */
+int bfvm_voxel_positions_noalloc(bfvm_ctx* ctx, int32_t count, const float* positions, int32_t* out3);
+
 /*
  * Measurement (no counterpart in the reference, whose only timing is std::chrono around whole phases,
  * src/main.cpp:73-101): enqueues `iterations` back-to-back launches of one kernel or phase on the context's

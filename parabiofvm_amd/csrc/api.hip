@@ -292,6 +292,7 @@ int bfvm_create(int32_t device, bfvm_ctx** out)
 		if (device < 0 || device >= count)
 			throw std::invalid_argument("device index out of range");
 		BFVM_HIP(hipSetDevice(device));
+		setup_noalloc_scratch(device);
 		auto* ctx = new bfvm_ctx();
 		ctx->device = device;
 		hipError_t err = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -1236,6 +1237,25 @@ int bfvm_voxel_positions(bfvm_ctx* ctx, int32_t count, const float* positions, i
 		}
 		BFVM_HIP(hipFree(d_pos));
 		BFVM_HIP(hipFree(d_out));
+		return BFVM_OK;
+	});
+}
+
+int bfvm_voxel_positions_noalloc(bfvm_ctx* ctx, int32_t count, const float* positions, int32_t* out3)
+{
+	return guarded("bfvm_voxel_positions_noalloc", [&] {
+		require_init(ctx);
+		if (count < 0 || (count > 0 && (!positions || !out3)))
+			throw std::invalid_argument("bad arguments");
+		if (count > noalloc_max_points)
+			throw std::invalid_argument("count " + std::to_string(count) + " exceeds the " +
+										std::to_string(noalloc_max_points) +
+										" points of the no-allocation scratch (use bfvm_voxel_positions)");
+		if (count == 0)
+			return BFVM_OK;
+		BFVM_HIP(hipSetDevice(ctx->device));
+		enter_ctx(ctx);
+		voxel_positions_noalloc(ctx, count, positions, out3);
 		return BFVM_OK;
 	});
 }

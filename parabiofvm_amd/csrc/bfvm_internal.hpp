@@ -529,6 +529,11 @@ void prepare_batch_gradient(bfvm_batch* b);
 void launch_batched_gradient(bfvm_batch* b);
 void free_batch_gradient(bfvm_batch* b);
 
+// entry point without allocation: the file-scope device scratch of noalloc.hip, looked up per device
+constexpr int noalloc_max_points = 1 << 20;
+void setup_noalloc_scratch(int device); // with `device` current, once per device (bfvm_create)
+void voxel_positions_noalloc(bfvm_ctx* ctx, int count, const float* positions, int32_t* out3);
+
 // autotuner: time candidate launch shapes on the context's mesh / the batch and keep the fastest
 void autotune_context(bfvm_ctx* ctx, bool tune_gradient);
 void autotune_batch(bfvm_batch* b);

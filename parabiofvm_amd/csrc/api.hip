@@ -684,6 +684,17 @@ int bfvm_set_gradient_tuning(bfvm_ctx* ctx, int32_t workgroups)
 	});
 }
 
+int bfvm_set_gradient_stacked(bfvm_ctx* ctx, int32_t stacked)
+{
+	return guarded("bfvm_set_gradient_stacked", [&] {
+		if (!ctx)
+			throw std::invalid_argument("null context");
+		ctx->gradient_stacked = stacked != 0;
+		ctx->epoch++;
+		return BFVM_OK;
+	});
+}
+
 int bfvm_autotune(bfvm_ctx* ctx)
 {
 	return guarded("bfvm_autotune", [&] {

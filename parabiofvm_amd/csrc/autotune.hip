@@ -29,7 +29,12 @@ using shape_key = std::tuple<int, int, int, int, int, int, int>; // axis (or -1:
 
 std::mutex g_cache_mutex;
 std::map<shape_key, sweep_shape> g_sweep_cache;
-std::map<shape_key, int> g_gradient_cache;
+struct gradient_shape
+{
+	int groups = 0;
+	bool stacked = false;
+};
+std::map<shape_key, gradient_shape> g_gradient_cache;
 std::map<std::string, bfvm_batch::tuning_t> g_batch_cache; // key: storage and every instance's shape
 
 shape_key key_of(const bfvm_ctx* ctx, int axis)
@@ -155,8 +160,9 @@ bool apply_cached_batch_tuning(bfvm_batch* b)
 
 // Text file, one entry per line:
 //   sweep <axis> <X> <Y> <Z> <S> <storage> <algorithm> <chunk> <lines> <workgroups>
-//   gradient <X> <Y> <Z> <S> <storage> <algorithm> <workgroups>
-//   batch <key> <group threads [3][2]> <max blocks [3][2]> <gradient tiles> <gradient blocks>
+//   gradient <X> <Y> <Z> <S> <storage> <algorithm> <workgroups> <stacked>
+//   batch <key> <group threads [3][2]> <max blocks [3][2]> <gradient tiles> <gradient blocks> <gradient rows>
+//         <gradient z planes> <gradient split> <gradient stacked>
 void save_tuning(const char* path)
 {
 	std::lock_guard<std::mutex> lock(g_cache_mutex);
@@ -167,8 +173,8 @@ void save_tuning(const char* path)
 		std::fprintf(f, "sweep %d %d %d %d %d %d %d %d %d %d\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
 					 std::get<3>(k), std::get<4>(k), std::get<5>(k), std::get<6>(k), v.chunk, v.lines, v.workgroups);
 	for (const auto& [k, v] : g_gradient_cache)
-		std::fprintf(f, "gradient %d %d %d %d %d %d %d\n", std::get<1>(k), std::get<2>(k), std::get<3>(k),
-					 std::get<4>(k), std::get<5>(k), std::get<6>(k), v);
+		std::fprintf(f, "gradient %d %d %d %d %d %d %d %d\n", std::get<1>(k), std::get<2>(k), std::get<3>(k),
+					 std::get<4>(k), std::get<5>(k), std::get<6>(k), v.groups, v.stacked ? 1 : 0);
 	for (const auto& [k, t] : g_batch_cache)
 	{
 		std::fprintf(f, "batch %s", k.c_str());
@@ -178,7 +184,8 @@ void save_tuning(const char* path)
 		for (int a = 0; a < 3; a++)
 			for (int w = 0; w < 2; w++)
 				std::fprintf(f, " %d", t.max_blocks[a][w]);
-		std::fprintf(f, " %d %d\n", t.gradient_tiles, t.gradient_blocks);
+		std::fprintf(f, " %d %d %d %d %d %d\n", t.gradient_tiles, t.gradient_blocks, t.gradient_rows, t.gradient_zchunk,
+					 t.gradient_split, t.gradient_stacked);
 	}
 	const bool bad = std::ferror(f) != 0;
 	std::fclose(f);
@@ -220,10 +227,10 @@ void load_tuning(const char* path)
 		}
 		else if (kind == "gradient")
 		{
-			int X, Y, Z, S, st, alg, groups;
-			ok = static_cast<bool>(in >> X >> Y >> Z >> S >> st >> alg >> groups);
+			int X, Y, Z, S, st, alg, groups, stacked;
+			ok = static_cast<bool>(in >> X >> Y >> Z >> S >> st >> alg >> groups >> stacked);
 			if (ok)
-				g_gradient_cache[shape_key { -1, X, Y, Z, S, st, alg }] = groups;
+				g_gradient_cache[shape_key { -1, X, Y, Z, S, st, alg }] = gradient_shape { groups, stacked != 0 };
 		}
 		else if (kind == "batch")
 		{
@@ -236,7 +243,9 @@ void load_tuning(const char* path)
 			for (int a = 0; a < 3 && ok; a++)
 				for (int w = 0; w < 2 && ok; w++)
 					ok = static_cast<bool>(in >> t.max_blocks[a][w]);
-			ok = ok && static_cast<bool>(in >> t.gradient_tiles >> t.gradient_blocks);
+			ok = ok && static_cast<bool>(in >> t.gradient_tiles >> t.gradient_blocks >> t.gradient_rows >>
+										  t.gradient_zchunk >> t.gradient_split >> t.gradient_stacked) &&
+				 (t.gradient_rows == 4 || t.gradient_rows == 8);
 			if (ok)
 				g_batch_cache[key] = t;
 		}
@@ -276,30 +285,37 @@ void autotune_context(bfvm_ctx* ctx, bool tune_gradient)
 	if (tune_gradient && ctx->mesh.dims == 3)
 	{
 		const shape_key key = key_of(ctx, -1);
-		int groups = -1;
+		gradient_shape best;
+		bool cached = false;
 		{
 			std::lock_guard<std::mutex> lock(g_cache_mutex);
 			auto it = g_gradient_cache.find(key);
 			if (it != g_gradient_cache.end())
-				groups = it->second;
+			{
+				best = it->second;
+				cached = true;
+			}
 		}
-		if (groups < 0)
+		if (!cached)
 		{
 			float best_t = 1e30f;
-			for (int g : { 4096, 8192, 16384, 32768, 65536 })
-			{
-				ctx->gradient_groups = g;
-				const float t = best_ms(ctx->stream, reps, [&] { launch_gradient(ctx); });
-				if (t < best_t)
+			for (bool stacked : { false, true })
+				for (int g : { 2048, 4096, 8192, 16384, 32768, 65536 })
 				{
-					best_t = t;
-					groups = g;
+					ctx->gradient_groups = g;
+					ctx->gradient_stacked = stacked;
+					const float t = best_ms(ctx->stream, reps, [&] { launch_gradient(ctx); });
+					if (t < best_t)
+					{
+						best_t = t;
+						best = gradient_shape { g, stacked };
+					}
 				}
-			}
 			std::lock_guard<std::mutex> lock(g_cache_mutex);
-			g_gradient_cache[key] = groups;
+			g_gradient_cache[key] = best;
 		}
-		ctx->gradient_groups = groups;
+		ctx->gradient_groups = best.groups;
+		ctx->gradient_stacked = best.stacked;
 	}
 	BFVM_HIP(hipStreamSynchronize(ctx->stream));
 	ctx->epoch++;
@@ -356,6 +372,35 @@ void autotune_batch(bfvm_batch* b)
 	}
 	if (b->grad.nitems[2] > 0)
 	{
+		// tile shape first (rows per thread, z planes per tile), then the work distribution
+		int best_rows = b->tuning.gradient_rows, best_zc = b->tuning.gradient_zchunk;
+		int best_split = b->tuning.gradient_split, best_stacked = b->tuning.gradient_stacked;
+		float best_shape = 1e30f;
+		for (int stacked = 0; stacked < 2; stacked++)
+			for (int split = 0; split < (b->storage == BFVM_STORAGE_F32 ? 2 : 1); split++)
+				for (int rows : { 4, 8 })
+					for (int zc : { 16, 32, 64 })
+					{
+						b->tuning.gradient_stacked = stacked;
+						b->tuning.gradient_split = split;
+						b->tuning.gradient_rows = rows;
+						b->tuning.gradient_zchunk = zc;
+						free_batch_gradient(b);
+						prepare_batch_gradient(b);
+						const float t = best_ms(b->stream, reps, [&] { launch_batched_gradient(b); });
+						if (t < best_shape)
+						{
+							best_shape = t;
+							best_rows = rows;
+							best_zc = zc;
+							best_split = split;
+							best_stacked = stacked;
+						}
+					}
+		b->tuning.gradient_stacked = best_stacked;
+		b->tuning.gradient_split = best_split;
+		b->tuning.gradient_rows = best_rows;
+		b->tuning.gradient_zchunk = best_zc;
 		int best_tiles = b->tuning.gradient_tiles, best_blocks = b->tuning.gradient_blocks;
 		float best_t = 1e30f;
 		for (int tiles : { 1, 2, 4, 8 })

@@ -409,6 +409,7 @@ struct bfvm_ctx
 	int chunk_request[3] = { 0, 0, 0 };
 	int lines_request[3] = { 0, 0, 0 };
 	int gradient_groups = 0; // workgroups the 3-D gradient aims for (0 = automatic)
+	bool gradient_stacked = false; // 3-D gradient tiles with the workgroup's waves stacked along y
 	int sweep_workgroups[3] = { 0, 0, 0 }; // most workgroups of a sweep launch per axis (0 = the built-in cap)
 
 	bfvm::agents_buffers ag;
@@ -476,13 +477,17 @@ struct bfvm_batch
 	} cell;
 	// launch shapes of the work-list kernels (bfvm_batch_autotune picks them): per axis and thread layout
 	// (4-element vectors, one element) the threads a line group aims for and the workgroup cap; for the
-	// gradient the tiles per work item and the workgroup cap
+	// gradient the y rows per thread, the z planes per tile, the tiles per work item and the workgroup cap
 	struct tuning_t
 	{
 		int group_threads[3][2] = { { 256, 256 }, { 256, 256 }, { 256, 256 } };
 		int max_blocks[3][2] = { { 2048, 2048 }, { 2048, 2048 }, { 2048, 2048 } };
 		int gradient_tiles = 4;
 		int gradient_blocks = 8192;
+		int gradient_rows = 4;	  // y rows per thread (4 or 8)
+		int gradient_zchunk = 16; // z planes per gradient tile
+		int gradient_split = 0;	  // FP32: one gradient launch per thread layout instead of one for both
+		int gradient_stacked = 0; // tiles with the workgroup's waves stacked along y
 	} tuning;
 	uint64_t epoch = 0;
 	bfvm::graph_slot graphs[4];

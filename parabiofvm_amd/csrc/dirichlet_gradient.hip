@@ -205,7 +205,6 @@ __global__ __launch_bounds__(256) void gradient_kernel(float* __restrict__ grad,
 constexpr int grad_tile = 256;
 constexpr int num_xcds = 8;
 
-constexpr int batch_gradient_rows = 4;
 
 
 // One 3-D gradient field: the densities, the output and the tiling of its (y, q) columns.
@@ -386,12 +385,151 @@ __device__ __forceinline__ void gradient_tile(const gradient_args& a, int tile)
 	}
 }
 
-template <int V, int K>
+// The same gradient with the four waves of a workgroup stacked along y: each wave takes a 64-column segment
+// (q) and TY rows, the waves consecutive TY-row bands of that segment.  The rows where two waves meet are
+// passed through LDS (each wave publishes its first and last row of the plane, double-buffered by plane parity,
+// one barrier per plane), so a workgroup reads its 4 TY rows plus 2 halo rows per plane instead of TY + 2 per
+// TY rows, and partly filled tiles at a row's end idle at most one wave-width of lanes.  Results are the same
+// expressions on the same values as gradient_tile, bit for bit.  Every wave runs every barrier, also waves
+// whose rows or columns lie outside the mesh.
+template <int V, int K, int TY>
+__device__ __forceinline__ void gradient_tile_stacked(const gradient_args& a, int tile, typename vec_of<V>::type* sh)
+{
+	using vec = typename vec_of<V>::type;
+	const int zc = tile % a.zchunks;
+	const int row_tile = tile / a.zchunks;
+	const int wave = threadIdx.x / wave_lanes;
+	const int lane = threadIdx.x & (wave_lanes - 1);
+	const int y0 = (row_tile / a.tiles_per_row) * (4 * TY) + wave * TY;
+	const int q = ((row_tile % a.tiles_per_row) * wave_lanes + lane) * V;
+	const int S = a.S, XS = a.XS, Y = a.Y, Z = a.Z;
+	const bool col = q < XS;
+	const int ny = max(0, min(TY, Y - y0)); // rows of this wave (0: outside the mesh, barriers only)
+	const bool run = col && ny > 0;
+	const int z0 = zc * a.zchunk;
+	const int z1 = min(Z, z0 + a.zchunk);
+	const int64_t plane = (int64_t)Y * XS;
+	const int64_t off = (int64_t)max(0, min(y0, Y - 1)) * XS + (col ? q : 0);
+	float* g0 = a.grad + off;
+	float* g1 = g0 + a.VS;
+	float* g2 = g1 + a.VS;
+	const bool xlo = q < S, xhi = q >= XS - S;
+	const int SG = S / V;
+	const float bx = a.bx, cx = a.cx, by = a.by, cy = a.cy, bz = a.bz, cz = a.cz;
+	const void* dens = a.dens;
+	// neighbours in y: the wave below / above when it has rows, else the plane's halo row in memory
+	const bool below_wave = wave > 0;
+	const bool above_wave = wave < 3 && y0 + ny < Y && ny == TY;
+
+	vec is = vec(1.f);
+	if constexpr (K != store_f32)
+		if (col)
+			is = vload<V>(a.inv + q % S);
+	auto ld = [&](int64_t o) {
+		const vec d = dload<K, V>(dens, off + o);
+		if constexpr (K == store_f32)
+			return d;
+		else
+			return d * is;
+	};
+	vec zm[TY], c[TY];
+#pragma unroll
+	for (int r = 0; r < TY; r++)
+	{
+		zm[r] = (run && r < ny && z0 > 0) ? ld((int64_t)(z0 - 1) * plane + (int64_t)r * XS) : vec(0.f);
+		c[r] = (run && r < ny) ? ld((int64_t)z0 * plane + (int64_t)r * XS) : vec(0.f);
+	}
+	for (int z = z0; z < z1; z++)
+	{
+		const int64_t o = (int64_t)z * plane;
+		// publish this wave's first and last row of plane z
+		vec* buf = sh + (z & 1) * (2 * 4 * wave_lanes);
+		buf[(2 * wave) * wave_lanes + lane] = c[0];
+		buf[(2 * wave + 1) * wave_lanes + lane] = c[ny > 0 ? ny - 1 : 0];
+		vec zp[TY];
+#pragma unroll
+		for (int r = 0; r < TY; r++)
+			zp[r] = (run && r < ny && z + 1 < Z) ? ld(o + plane + (int64_t)r * XS) : vec(0.f);
+		vec below = vec(0.f), above = vec(0.f);
+		if (run && !below_wave && y0 > 0)
+			below = ld(o - XS);
+		if (run && !above_wave && y0 + ny < Y)
+			above = ld(o + (int64_t)ny * XS);
+		__syncthreads();
+		if (run)
+		{
+			if (below_wave)
+				below = buf[(2 * (wave - 1) + 1) * wave_lanes + lane];
+			if (above_wave)
+				above = buf[(2 * (wave + 1)) * wave_lanes + lane];
+		}
+#pragma unroll
+		for (int r = 0; r < TY; r++)
+		{
+			// x neighbours across lanes (every lane of the wave takes part in the exchange)
+			vec xp = shfl_vec<V>(c[r], lane + SG);
+			vec xm = shfl_vec<V>(c[r], lane - SG);
+			if (!run || r >= ny)
+				continue;
+			const int y = y0 + r;
+			const int64_t orow = o + (int64_t)r * XS;
+			if (xhi)
+				xp = vec(0.f);
+			else if (lane + SG >= wave_lanes)
+				xp = ld(orow + S);
+			if (xlo)
+				xm = vec(0.f);
+			else if (lane < SG)
+				xm = ld(orow - S);
+			const vec ym = r == 0 ? below : c[r - 1 < 0 ? 0 : r - 1];
+			const vec yp = r == ny - 1 ? above : c[r + 1 < TY ? r + 1 : TY - 1];
+			vec gx, gy, gz;
+			if (xlo)
+				gx = (xp - c[r]) * bx;
+			else if (xhi)
+				gx = (c[r] - xm) * bx;
+			else
+				gx = (xp - xm) * cx;
+			if (y == 0)
+				gy = (yp - c[r]) * by;
+			else if (y == Y - 1)
+				gy = (c[r] - ym) * by;
+			else
+				gy = (yp - ym) * cy;
+			if (z == 0)
+				gz = (zp[r] - c[r]) * bz;
+			else if (z == Z - 1)
+				gz = (c[r] - zm[r]) * bz;
+			else
+				gz = (zp[r] - zm[r]) * cz;
+			const int64_t go = o + (int64_t)r * XS;
+			__builtin_nontemporal_store(gx, (vec*)(g0 + go));
+			__builtin_nontemporal_store(gy, (vec*)(g1 + go));
+			__builtin_nontemporal_store(gz, (vec*)(g2 + go));
+		}
+#pragma unroll
+		for (int r = 0; r < TY; r++)
+		{
+			zm[r] = c[r];
+			c[r] = zp[r];
+		}
+	}
+	__syncthreads(); // the next tile of this workgroup writes the same LDS rows
+}
+
+template <int V, int K, bool STACKED>
 __global__ __launch_bounds__(grad_tile) void gradient3d_kernel(gradient_args a, int per_xcd)
 {
 	const int b = blockIdx.x;
 	const int tile = (b % num_xcds) * per_xcd + b / num_xcds;
-	if (tile < a.ntiles)
+	if (tile >= a.ntiles)
+		return;
+	if constexpr (STACKED)
+	{
+		__shared__ typename vec_of<V>::type sh[2 * 2 * 4 * wave_lanes];
+		gradient_tile_stacked<V, K, gradient_rows<V, K>()>(a, tile, sh);
+	}
+	else
 		gradient_tile<V, K, gradient_rows<V, K>()>(a, tile);
 }
 
@@ -400,22 +538,33 @@ __global__ __launch_bounds__(grad_tile) void gradient3d_kernel(gradient_args a, 
 // time (one launch for the whole batch); V = 4 or 1 serves one layout, so that layout gets its own register
 // budget.  Measured on the 64-instance batch: FP32 4.38 ms with one launch against 4.46 ms split by layout,
 // 16-bit 4.25 against 4.14 ms (the 16-bit one-element instances need fewer registers than the 4-wide ones).
-template <int K, int V>
+// TY: y rows per thread (4 or 8, a tuning choice: 8 rows share their y neighbours over more outputs but hold
+// more registers).
+template <int K, int V, int TY, bool STACKED>
 __global__ __launch_bounds__(grad_tile) void gradient_batched_kernel(const gradient_args* __restrict__ inst,
 																	 const batch_item* __restrict__ items, int nitems)
 {
+	using vec4 = typename vec_of<4>::type;
+	__shared__ vec4 sh[STACKED ? 2 * 2 * 4 * wave_lanes : 1];
 	for (int it = blockIdx.x; it < nitems; it += gridDim.x)
 	{
 		const batch_item w = items[it];
 		const gradient_args a = inst[w.instance];
 		for (int64_t t = w.begin; t < w.end; t++)
 		{
-			if constexpr (V != 0)
-				gradient_tile<V, K, batch_gradient_rows>(a, (int)t);
+			if constexpr (STACKED)
+			{
+				if (V == 4 || (V == 0 && a.vec == 4))
+					gradient_tile_stacked<4, K, TY>(a, (int)t, sh);
+				else
+					gradient_tile_stacked<1, K, TY>(a, (int)t, (float*)sh);
+			}
+			else if constexpr (V != 0)
+				gradient_tile<V, K, TY>(a, (int)t);
 			else if (a.vec == 4)
-				gradient_tile<4, K, batch_gradient_rows>(a, (int)t);
+				gradient_tile<4, K, TY>(a, (int)t);
 			else
-				gradient_tile<1, K, batch_gradient_rows>(a, (int)t);
+				gradient_tile<1, K, TY>(a, (int)t);
 		}
 	}
 }
@@ -445,8 +594,8 @@ void launch_copy(bfvm_ctx* ctx)
 }
 
 // Tiling of one 3-D field with `rows` y rows per thread: z is split so that there are about `target` tiles
-// (workgroups).
-static gradient_args make_gradient_args(const bfvm_ctx* ctx, int64_t target, int rows)
+// (workgroups).  Stacked tiles: 64 columns by 4 * rows rows per workgroup; otherwise 256 columns by `rows`.
+static gradient_args make_gradient_args(const bfvm_ctx* ctx, int64_t target, int rows, bool stacked = false)
 {
 	const auto& m = ctx->mesh;
 	gradient_args a;
@@ -458,8 +607,10 @@ static gradient_args make_gradient_args(const bfvm_ctx* ctx, int64_t target, int
 	a.S = m.S;
 	a.XS = m.grid[0] * m.S;
 	a.vec = m.S % 4 == 0 ? 4 : 1;
-	a.tiles_per_row = (a.XS / a.vec + grad_tile - 1) / grad_tile;
-	const int64_t row_tiles = (int64_t)((a.Y + rows - 1) / rows) * a.tiles_per_row;
+	const int width = stacked ? wave_lanes : grad_tile;
+	const int band = stacked ? 4 * rows : rows;
+	a.tiles_per_row = (a.XS / a.vec + width - 1) / width;
+	const int64_t row_tiles = (int64_t)((a.Y + band - 1) / band) * a.tiles_per_row;
 	int zchunks = (int)std::max<int64_t>(1, std::min<int64_t>(a.Z, (target + row_tiles - 1) / row_tiles));
 	a.zchunk = (a.Z + zchunks - 1) / zchunks;
 	a.zchunks = (a.Z + a.zchunk - 1) / a.zchunk;
@@ -501,10 +652,12 @@ void prepare_batch_gradient(bfvm_batch* b)
 			args[i] = gradient_args {};
 			continue;
 		}
-		// tiles of 16 z planes: ask for (row tiles) x ceil(Z / 16)
-		const gradient_args probe = make_gradient_args(ctx, 1, batch_gradient_rows);
+		// tiles of `zc` z planes: ask for (row tiles) x ceil(Z / zc)
+		const int rows = b->tuning.gradient_rows, zc = std::max(1, b->tuning.gradient_zchunk);
+		const bool stacked = b->tuning.gradient_stacked != 0;
+		const gradient_args probe = make_gradient_args(ctx, 1, rows, stacked);
 		const int64_t row_tiles = (int64_t)probe.ntiles;
-		args[i] = make_gradient_args(ctx, row_tiles * ((ctx->mesh.grid[2] + 15) / 16), batch_gradient_rows);
+		args[i] = make_gradient_args(ctx, row_tiles * ((ctx->mesh.grid[2] + zc - 1) / zc), rows, stacked);
 		const int m = args[i].vec == 4 ? 1 : 0;
 		const int per_item = std::max(1, b->tuning.gradient_tiles);
 		for (int64_t t0 = 0; t0 < args[i].ntiles; t0 += per_item)
@@ -529,26 +682,39 @@ void prepare_batch_gradient(bfvm_batch* b)
 	}
 }
 
-void launch_batched_gradient(bfvm_batch* b)
+template <int TY, bool STACKED>
+static void launch_batched_gradient_t(bfvm_batch* b)
 {
 	const auto& gp = b->grad;
 	by_storage(b->storage, [&](auto kind) {
 		constexpr int K = decltype(kind)::value;
-		if constexpr (K == store_f32)
+		// FP32: one launch over both thread layouts unless tuning split it; 16-bit: one launch per layout
+		if (K == store_f32 && !b->tuning.gradient_split)
 		{
 			if (gp.nitems[2])
-				hipLaunchKernelGGL((gradient_batched_kernel<K, 0>), dim3(gp.blocks[2]), dim3(grad_tile), 0, b->stream,
-								   (const gradient_args*)gp.inst, gp.items[2], gp.nitems[2]);
+				hipLaunchKernelGGL((gradient_batched_kernel<K, 0, TY, STACKED>), dim3(gp.blocks[2]), dim3(grad_tile), 0,
+								   b->stream, (const gradient_args*)gp.inst, gp.items[2], gp.nitems[2]);
 			return;
 		}
 		if (gp.nitems[1])
-			hipLaunchKernelGGL((gradient_batched_kernel<K, 4>), dim3(gp.blocks[1]), dim3(grad_tile), 0, b->stream,
+			hipLaunchKernelGGL((gradient_batched_kernel<K, 4, TY, STACKED>), dim3(gp.blocks[1]), dim3(grad_tile), 0,
+							   b->stream,
 							   (const gradient_args*)gp.inst, gp.items[1], gp.nitems[1]);
 		if (gp.nitems[0])
-			hipLaunchKernelGGL((gradient_batched_kernel<K, 1>), dim3(gp.blocks[0]), dim3(grad_tile), 0, b->stream,
+			hipLaunchKernelGGL((gradient_batched_kernel<K, 1, TY, STACKED>), dim3(gp.blocks[0]), dim3(grad_tile), 0,
+							   b->stream,
 							   (const gradient_args*)gp.inst, gp.items[0], gp.nitems[0]);
 	});
 	BFVM_HIP(hipGetLastError());
+}
+
+void launch_batched_gradient(bfvm_batch* b)
+{
+	const bool st = b->tuning.gradient_stacked != 0;
+	if (b->tuning.gradient_rows == 8)
+		st ? launch_batched_gradient_t<8, true>(b) : launch_batched_gradient_t<8, false>(b);
+	else
+		st ? launch_batched_gradient_t<4, true>(b) : launch_batched_gradient_t<4, false>(b);
 }
 
 void launch_gradient(bfvm_ctx* ctx)
@@ -567,9 +733,12 @@ void launch_gradient(bfvm_ctx* ctx)
 		by_storage(ctx->storage, [&](auto k) {
 			constexpr int K = decltype(k)::value;
 			const bool v4 = m.S % 4 == 0;
-			const gradient_args a = make_gradient_args(ctx, target, v4 ? gradient_rows<4, K>() : gradient_rows<1, K>());
+			const bool st = ctx->gradient_stacked;
+			const gradient_args a =
+				make_gradient_args(ctx, target, v4 ? gradient_rows<4, K>() : gradient_rows<1, K>(), st);
 			const int per_xcd = (a.ntiles + num_xcds - 1) / num_xcds;
-			auto kernel = v4 ? gradient3d_kernel<4, K> : gradient3d_kernel<1, K>;
+			auto kernel = v4 ? (st ? gradient3d_kernel<4, K, true> : gradient3d_kernel<4, K, false>)
+							 : (st ? gradient3d_kernel<1, K, true> : gradient3d_kernel<1, K, false>);
 			hipLaunchKernelGGL(kernel, dim3((unsigned)(per_xcd * num_xcds)), dim3(grad_tile), 0, ctx->stream, a,
 							   per_xcd);
 		});

@@ -232,6 +232,10 @@ class DeviceSolver:
                                                                     ctypes.byref(lpg)))
         return p.value, m.value, lpg.value
 
+    def set_gradient_stacked(self, stacked):
+        """3-D gradient tiles with the workgroup's waves stacked along y (True) or side by side (False)."""
+        check("bfvm_set_gradient_stacked", self.lib.bfvm_set_gradient_stacked(self.h, int(bool(stacked))))
+
     def set_gradient_tuning(self, workgroups):
         check("bfvm_set_gradient_tuning", self.lib.bfvm_set_gradient_tuning(self.h, int(workgroups)))
 

@@ -64,15 +64,15 @@ def test_tuning_cache_file_round_trip(tmp_path):
     from parabiofvm_amd import _native
 
     src = tmp_path / "in.txt"
-    src.write_text("sweep 2 400 400 400 8 0 0 16 16 4096\ngradient 400 400 400 8 0 0 8192\n"
-                   "batch 0:1;32,40,50,3,3 256 128 256 256 512 256 2048 2048 1024 2048 4096 2048 4 8192\n")
+    src.write_text("sweep 2 400 400 400 8 0 0 16 16 4096\ngradient 400 400 400 8 0 0 8192 1\n"
+                   "batch 0:1;32,40,50,3,3 256 128 256 256 512 256 2048 2048 1024 2048 4096 2048 4 8192 8 32 1 1\n")
     _native.tuning_load(str(src))
     out = tmp_path / "out.txt"
     _native.tuning_save(str(out))
     lines = set(out.read_text().splitlines())
     assert "sweep 2 400 400 400 8 0 0 16 16 4096" in lines
-    assert "gradient 400 400 400 8 0 0 8192" in lines
-    assert "batch 0:1;32,40,50,3,3 256 128 256 256 512 256 2048 2048 1024 2048 4096 2048 4 8192" in lines
+    assert "gradient 400 400 400 8 0 0 8192 1" in lines
+    assert "batch 0:1;32,40,50,3,3 256 128 256 256 512 256 2048 2048 1024 2048 4096 2048 4 8192 8 32 1 1" in lines
     bad = tmp_path / "bad.txt"
     bad.write_text("sweep 2 400\n")
     lib = _native.load()

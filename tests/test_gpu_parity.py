@@ -251,6 +251,36 @@ def test_gpu_gradient_bitwise(name, dims, shape, S, pkg, oracle_mod):
     assert_bitwise(g.gradients, o.gradients, "gradients")
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+@pytest.mark.parametrize("shape,S", [((3, 4, 5), 2), ((70, 45, 33), 3), ((64, 40, 36), 8), ((100, 37, 9), 16),
+                                     ((17, 2, 9), 4), ((130, 71, 6), 1)])
+def test_gpu_gradient_stacked_tiles(shape, S, storage, pkg, oracle_mod):
+    """The gradient's stacked tile layout (waves stacked along y, the rows where they meet passed through LDS) on
+    meshes whose rows and columns do not fill whole tiles: bit-identical to the float oracle (FP32 storage) and
+    to the other tile layout (FP16 storage), at several workgroup targets (z split into chunks of different
+    lengths, so tiles start and end at every plane parity)."""
+    p = random_problem(3, shape, S, seed=45)
+    o, g = build_pair(pkg, oracle_mod, p)
+    if storage != "f32":
+        g.set_storage(storage)
+        g.initialize()
+    g.store()
+    o.gradient()
+    results = []
+    for stacked in (False, True):
+        for groups in (1, 3, 64, 4096):
+            g.set_gradient_stacked(stacked)
+            g.set_gradient_tuning(groups)
+            g.gradients[:] = np.nan
+            g.gradient()
+            g.load()
+            if storage == "f32":
+                assert_bitwise(g.gradients, o.gradients, f"stacked {stacked} groups {groups}")
+            results.append(g.gradients.copy())
+    for r in results[1:]:
+        assert_bitwise(r, results[0], "tile layouts and z chunkings")
+
+
 @pytest.mark.parametrize("name,dims,shape,S", SHAPES, ids=[s[0] for s in SHAPES])
 def test_gpu_dirichlet_bitwise(name, dims, shape, S, pkg, oracle_mod):
     p = random_problem(dims, shape, S, seed=43)

@@ -97,7 +97,8 @@ sweep_shape search_sweep(bfvm_ctx* ctx, int axis, int reps)
 	std::vector<int> chunks = { 16, 32 };
 	if (ctx->storage == BFVM_STORAGE_F32)
 		chunks.push_back(64);
-	const std::vector<int> lines = axis == 0 ? std::vector<int> { 0, 1, 2, 4 } : std::vector<int> { 0, 4, 8, 16, 32 };
+	const std::vector<int> lines =
+		axis == 0 ? std::vector<int> { 0, 1, 2, 4, 8 } : std::vector<int> { 0, 4, 8, 16, 24, 32 };
 	sweep_shape best;
 	float best_t = 1e30f;
 	for (int c : chunks)

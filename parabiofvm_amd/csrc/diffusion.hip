@@ -1090,8 +1090,8 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 	}
 }
 
-template <int M, int AXIS, int V, int K>
-__global__ __launch_bounds__(max_group_threads(V)) void sweep_partitioned_kernel(part_args a)
+template <int M, int AXIS, int V, int K, int MAXT = max_group_threads(V)>
+__global__ __launch_bounds__(MAXT) void sweep_partitioned_kernel(part_args a)
 {
 	extern __shared__ float lds[];
 	sweep_groups<M, AXIS, V, K>(a, lds, blockIdx.x, a.groups, gridDim.x);
@@ -1505,11 +1505,21 @@ static void free_partition_tables(axis_tables& t)
 	t.coef = t.red = t.A = nullptr;
 }
 
+// Most threads of a register-sweep workgroup: 4-substrate sweeps without the 16-bit prefetch fit 1024 threads
+// (124-128 VGPRs), with it they keep 512 (160 VGPRs); one substrate per thread always fits 1024.
+static int partition_max_threads(int vec, int storage, int axis)
+{
+	if (vec != 4)
+		return 1024;
+	return (storage == BFVM_STORAGE_F32 || axis == 0) ? 1024 : 512;
+}
+
 // Lines handled by one workgroup and its thread count for a partition of P blocks.  `request` is the
 // number of lines per workgroup for y/z and the number of x rows per workgroup for x; 0 picks the shapes
 // measured fastest on 400^3 x 8 (x: as many rows as fit in 200 threads; y: 8 lines of 4-substrate vectors,
 // z: 16; one substrate per thread: 32).
-static void group_shape(int axis, int SG, int P, int request, int max_threads, int& lpg, int& threads)
+static void group_shape(int axis, int SG, int P, int request, int max_threads, int& lpg, int& threads,
+						bool vec4 = false)
 {
 	if (axis == 0)
 	{
@@ -1520,7 +1530,7 @@ static void group_shape(int axis, int SG, int P, int request, int max_threads, i
 	}
 	else
 	{
-		lpg = request > 0 ? request : (max_threads >= 1024 ? 32 : axis == 1 ? 8 : 16);
+		lpg = request > 0 ? request : (!vec4 && max_threads >= 1024 ? 32 : axis == 1 ? 8 : 16);
 		if (lpg * P > max_threads)
 			lpg = std::max(8, max_threads / P / 8 * 8);
 		threads = ((lpg * P + 63) / 64) * 64;
@@ -1567,8 +1577,11 @@ partition choose_partition(const bfvm_ctx* ctx, int axis, int chunk, int lines_r
 		p.ntypes = p.P == 1 ? 1 : 4;
 		p.mb = p.P == 1 ? n : p.base + 1;
 		int lpg, threads;
-		const int max_threads = p.vec == 4 ? 512 : 1024;
-		group_shape(axis, S / p.vec, p.P, lines_request, max_threads, lpg, threads);
+		// staged and batched sweeps (max_vec 1) keep their own limits; the register sweeps may widen (see
+		// partition_max_threads)
+		const int max_threads = max_vec >= 4 ? partition_max_threads(p.vec, ctx->storage, axis)
+											 : (p.vec == 4 ? 512 : 1024);
+		group_shape(axis, S / p.vec, p.P, lines_request, max_threads, lpg, threads, p.vec == 4);
 		p.lpg = lpg;
 		p.threads = threads;
 		const bool fits = threads <= max_threads && lpg * p.P <= threads &&
@@ -1648,6 +1661,10 @@ static part_args make_part_args(const sweep_geometry& g, const partition& p, con
 	return a;
 }
 
+// the 1024-thread instantiation exists where partition_max_threads allows it
+template <int V, int K, int AXIS>
+constexpr bool wide_group_ok() { return V == 4 && (K == store_f32 || AXIS == 0); }
+
 template <int M, int AXIS, int V, int K>
 static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const axis_tables& t)
 {
@@ -1657,6 +1674,18 @@ static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const a
 	const size_t lds = partition_lds_bytes(p, g.S, p.lpg);
 	const int64_t cap = ctx->sweep_workgroups[AXIS] > 0 ? ctx->sweep_workgroups[AXIS] : sweep_max_blocks;
 	const int blocks = (int)std::min<int64_t>(a.groups, cap);
+	if constexpr (wide_group_ok<V, K, AXIS>())
+	{
+		if (threads > max_group_threads(V))
+		{
+			hipLaunchKernelGGL((sweep_partitioned_kernel<M, AXIS, V, K, 1024>), dim3(blocks), dim3(threads), lds,
+							   ctx->stream, a);
+			BFVM_HIP(hipGetLastError());
+			return;
+		}
+	}
+	if (threads > max_group_threads(V))
+		throw std::logic_error("partitioned sweep: workgroup larger than the kernel allows");
 	hipLaunchKernelGGL((sweep_partitioned_kernel<M, AXIS, V, K>), dim3(blocks), dim3(threads), lds, ctx->stream, a);
 	BFVM_HIP(hipGetLastError());
 }

@@ -423,6 +423,26 @@ def test_gpu_partitioned_tuning(axis, chunk, lines, pkg, oracle_mod):
     assert_close(g.densities, o.densities, what=f"axis {axis} chunk {chunk} lines {lines}: {g.partition(axis)}")
 
 
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_gpu_wide_workgroups(axis, pkg, oracle_mod):
+    """4-substrate sweeps in workgroups of more than 512 threads (400-voxel lines, P = 25 blocks, 32 lines or x
+    rows per workgroup: 800 threads), with Dirichlet faces and interior voxels, against the float oracle."""
+    shape = [16, 12, 10]
+    shape[axis] = 400
+    p = random_problem(3, tuple(shape), 4, seed=67)
+    o, g = build_pair(pkg, oracle_mod, p)
+    g.set_sweep_tuning(axis, 16, 32)
+    P, M, lpg = g.partition(axis)
+    threads = lpg * P if axis else (lpg // 1) * P
+    assert threads > 512, (P, M, lpg)
+    g.store()
+    for _ in range(2):
+        o.diffusion()
+        g.diffusion()
+    g.load()
+    assert_close(g.densities, o.densities, what=f"axis {axis}: partition {(P, M, lpg)}")
+
+
 def test_gpu_sorted_reduction_agents_outside_and_reupload(pkg, oracle_mod):
     """Sorted reduction with agents outside the mesh (ignored), then a second upload that moves agents and
     switches back to the claim scheme: both uploads rebuild the agent -> voxel tables."""

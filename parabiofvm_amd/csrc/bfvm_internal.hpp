@@ -228,6 +228,54 @@ __device__ __forceinline__ void dstore_sr(void* base, int64_t idx, typename vec_
 	}
 }
 
+// Stochastic-rounding stores from a per-thread random stream, for code that stores a run of elements in a
+// fixed order (a sweep thread's register block): sr_seed hashes the run's first element index with the key
+// once, and every further pair of components takes the next word of a xorshift32 generator (Marsaglia 2003;
+// shifts 13, 17, 5) — three shift-xor pairs instead of a hash with 32-bit multiplies per element.  The state
+// is kept nonzero (zero is the generator's fixed point).
+__device__ __forceinline__ uint32_t xorshift32(uint32_t x)
+{
+	x ^= x << 13;
+	x ^= x >> 17;
+	x ^= x << 5;
+	return x;
+}
+
+__device__ __forceinline__ uint32_t sr_seed(int64_t idx, uint32_t key)
+{
+	return mix32((uint32_t)idx ^ ((uint32_t)((uint64_t)idx >> 32) * 0x9E3779B9u) ^ key) | 1u;
+}
+
+template <int K, int V>
+__device__ __forceinline__ void dstore_sr_stream(void* base, int64_t idx, typename vec_of<V>::type x, uint32_t& h)
+{
+	if constexpr (K == store_f32)
+		dstore<K, V>(base, idx, x);
+	else
+	{
+		using elem = typename store_of<K>::elem;
+		elem* p = (elem*)base + idx;
+		if constexpr (V == 1)
+		{
+			h = xorshift32(h);
+			gstore(p, sr_pair<K>(x, 0.f, h)[0]);
+		}
+		else
+		{
+			typename svec_of<K, V>::type y;
+#pragma unroll
+			for (int c = 0; c < V; c += 2)
+			{
+				h = xorshift32(h);
+				const auto o = sr_pair<K>(comp(x, c), comp(x, c + 1), h);
+				y[c] = o[0];
+				y[c + 1] = o[1];
+			}
+			gstore((typename svec_of<K, V>::type*)p, y);
+		}
+	}
+}
+
 // Calls f(std::integral_constant<int, K>) for the storage form K of a context.
 template <typename F>
 inline void by_storage(int storage, F&& f)

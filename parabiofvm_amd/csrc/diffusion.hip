@@ -646,8 +646,11 @@ __device__ __forceinline__ void twisted_solve(const float4* tab, float mu, int S
 }
 
 // Groups g_first, g_first + g_step, ... < g_end of one partitioned sweep, by one workgroup.  The caller
-// synchronises the workgroup before reusing `lds`.
-template <int M, int AXIS, int V, int K, bool PF = (K != store_f32 && AXIS != 0), bool NT = (K == store_f32 && AXIS != 0)>
+// synchronises the workgroup before reusing `lds`.  FULL: every block holds M - 1 or M elements
+// (a.base >= M - 1), so only the last element's presence differs between threads and the other per-element
+// guards are known at compile time.
+template <int M, int AXIS, int V, int K, bool PF = (K != store_f32 && AXIS != 0), bool NT = (K == store_f32 && AXIS != 0),
+		  bool FULL = false>
 __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int64_t g_first, int64_t g_end,
 											 int64_t g_step)
 {
@@ -691,6 +694,9 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 		mk = a.base + (k < a.rem ? 1 : 0);
 		type = (P == 1) ? 0 : (k == 0 ? 0 : (k == P - 1 ? 3 : (k < a.rem ? 1 : 2)));
 	}
+	// element j of the thread's block exists / is the block's last
+	auto has = [&](int j) { return (FULL && j < M - 1) || j < mk; };
+	auto last = [&](int j) { return (!FULL || j >= M - 2) && j == mk - 1; };
 	__syncthreads();
 
 	// PF (16-bit forms): load the next group's block while this one is solved.  A latency-bound sweep keeps a
@@ -710,7 +716,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 			const int64_t o = b + (int64_t)start * stride;
 #pragma unroll
 			for (int j = 0; j < M; j++)
-				if (j < mk)
+				if (has(j))
 					pre[j] = draw<K, V>(g.data, o + j * stride);
 		}
 	};
@@ -740,7 +746,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 			{
 #pragma unroll
 				for (int j = 0; j < M; j++)
-					if (j < mk)
+					if (has(j))
 						v[j] = dcvt<K, V>(pre[j]);
 			}
 			if (group + g_step < g_end)
@@ -752,7 +758,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 			{
 #pragma unroll
 				for (int j = 0; j < M; j++)
-					if (j < mk)
+					if (has(j))
 						v[j] = dload<K, V>(g.data, off0 + j * stride);
 			}
 			// 16-bit forms solve on the stored values: the scale is a power of two, so solving on scaled values and
@@ -763,7 +769,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 #pragma unroll
 			for (int j = 0; j < M; j++)
 			{
-				if (j < mk)
+				if (has(j))
 				{
 					const vec r = vload<V>(cr + j * S);
 					if (j < 2)
@@ -775,14 +781,14 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 			// backward elimination towards the block's last unknown
 #pragma unroll
 			for (int j = M - 3; j >= 1; j--)
-				if (j <= mk - 3)
+				if (has(j + 2))
 					v[j] = v[j] - vload<V>(cg + j * S) * v[j + 1];
-			if (mk >= 3)
+			if (has(2))
 				v[0] = vload<V>(ca) * (v[0] - vload<V>(cg) * v[1]);
 			rv[(2 * k) * lpg + line_local] = v[0];
 #pragma unroll
 			for (int j = 1; j < M; j++)
-				if (j == mk - 1)
+				if (last(j))
 					rv[(2 * k + 1) * lpg + line_local] = v[j];
 		}
 		__syncthreads();
@@ -830,16 +836,20 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 							line_face |= (cmin && coord[d] == 0) || (cmax && coord[d] == grid[d] - 1);
 					}
 			}
+			// the block's values are stored in element order, so the stochastic rounding draws its random words
+			// from one stream per block (a loop without the face checks for the blocks that meet no face was
+			// measured slower: the second copy of the loop costs registers)
 			const vec sc = vload<V>(lscale + s0);
+			uint32_t h = K == store_f32 ? 0u : sr_seed(off0, key);
 #pragma unroll
 			for (int j = 0; j < M; j++)
 			{
-				if (j < mk)
+				if (has(j))
 				{
 					vec x;
 					if (j == 0)
 						x = u0;
-					else if (j == mk - 1)
+					else if (last(j))
 						x = u1;
 					else
 						x = v[j] - vload<V>(ca + j * S) * u0 - vload<V>(cgp + j * S) * u1;
@@ -852,7 +862,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 					if constexpr (NT)
 						__builtin_nontemporal_store(x, (vec*)((float*)g.data + off0 + j * stride));
 					else
-						dstore_sr<K, V>(g.data, off0 + j * stride, x, key);
+						dstore_sr_stream<K, V>(g.data, off0 + j * stride, x, h);
 				}
 			}
 		}
@@ -863,7 +873,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 // sweep_groups for y/z lines whose threads hold V consecutive flattened plane positions that may wrap over
 // the substrates (V need not divide S): per-substrate tables in LDS extended periodically to S + V - 1 entries
 // and read component by component, per-component substrate and voxel coordinates for the faces.
-template <int M, int AXIS, int V, int K>
+template <int M, int AXIS, int V, int K, bool FULL = false>
 __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* lds, int64_t g_first,
 													 int64_t g_end, int64_t g_step)
 {
@@ -920,6 +930,8 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 		mk = a.base + (k < a.rem ? 1 : 0);
 		type = (P == 1) ? 0 : (k == 0 ? 0 : (k == P - 1 ? 3 : (k < a.rem ? 1 : 2)));
 	}
+	auto has = [&](int j) { return (FULL && j < M - 1) || j < mk; };
+	auto last = [&](int j) { return (!FULL || j >= M - 2) && j == mk - 1; };
 	__syncthreads();
 
 	// 16-bit forms load the next group's block while this one is solved (as in sweep_groups)
@@ -936,7 +948,7 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 			const int64_t o = b + (int64_t)start * stride;
 #pragma unroll
 			for (int j = 0; j < M; j++)
-				if (j < mk)
+				if (has(j))
 					pre[j] = draw<K, V>(g.data, o + j * stride);
 		}
 	};
@@ -966,7 +978,7 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 			{
 #pragma unroll
 				for (int j = 0; j < M; j++)
-					if (j < mk)
+					if (has(j))
 						v[j] = dcvt<K, V>(pre[j]);
 			}
 			if (group + g_step < g_end)
@@ -978,7 +990,7 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 			{
 #pragma unroll
 				for (int j = 0; j < M; j++)
-					if (j < mk)
+					if (has(j))
 						v[j] = dload<K, V>(g.data, off0 + j * stride);
 			}
 			// 16-bit forms solve on the stored values: the scale is a power of two, so solving on scaled values and
@@ -989,7 +1001,7 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 #pragma unroll
 			for (int j = 0; j < M; j++)
 			{
-				if (j < mk)
+				if (has(j))
 				{
 					const vec r = cld(cr + j * Sp);
 					if (j < 2)
@@ -1001,14 +1013,14 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 			// backward elimination towards the block's last unknown
 #pragma unroll
 			for (int j = M - 3; j >= 1; j--)
-				if (j <= mk - 3)
+				if (has(j + 2))
 					v[j] = v[j] - cld(cg + j * Sp) * v[j + 1];
-			if (mk >= 3)
+			if (has(2))
 				v[0] = cld(ca) * (v[0] - cld(cg) * v[1]);
 			rv[(2 * k) * lpg + line_local] = v[0];
 #pragma unroll
 			for (int j = 1; j < M; j++)
-				if (j == mk - 1)
+				if (last(j))
 					rv[(2 * k + 1) * lpg + line_local] = v[j];
 		}
 		__syncthreads();
@@ -1067,22 +1079,23 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 				}
 			}
 			const vec sc = cld(lscale + s0);
+			uint32_t h = K == store_f32 ? 0u : sr_seed(off0, key);
 #pragma unroll
 			for (int j = 0; j < M; j++)
 			{
-				if (j < mk)
+				if (has(j))
 				{
 					vec x;
 					if (j == 0)
 						x = u0;
-					else if (j == mk - 1)
+					else if (last(j))
 						x = u1;
 					else
 						x = v[j] - cld(ca + j * Sp) * u0 - cld(cgp + j * Sp) * u1;
 					const int i = start + j;
 					if (line_face || (i == 0 && end_min) || (i == a.n - 1 && end_max))
 						x = faces_slow<AXIS, V, true, K>(x, g, s0, coord, i, sc);
-					dstore_sr<K, V>(g.data, off0 + j * stride, x, key);
+					dstore_sr_stream<K, V>(g.data, off0 + j * stride, x, h);
 				}
 			}
 		}
@@ -1090,11 +1103,37 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 	}
 }
 
+// Waves per SIMD asked of the register allocator: the 4-substrate FP32 kernels and the 16-bit x kernel are
+// held to 128 VGPRs (4 waves); measured on 400^3 x 8 with full blocks: FP32 z 1.31 -> 0.99 ms, BF16 x 1.03 ->
+// 0.92 ms, while the 16-bit y/z kernels ran slower under a bound of 3 than without one.
+template <int V, int K, int AXIS>
+constexpr int sweep_waves_per_simd()
+{
+	return V == 4 && (K == store_f32 || AXIS == 0) ? 4 : 1;
+}
+
+// Blocks of M - 1 or M elements (the usual case for long lines) take the FULL form of the group loop.  Measured
+// on 400^3 x 8 BF16: y 1.02 -> 0.90 ms, z 1.09 -> 0.98 ms.
 template <int M, int AXIS, int V, int K, int MAXT = max_group_threads(V)>
-__global__ __launch_bounds__(MAXT) void sweep_partitioned_kernel(part_args a)
+__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(sweep_waves_per_simd<V, K, AXIS>())))
+void sweep_partitioned_kernel(part_args a)
 {
 	extern __shared__ float lds[];
-	sweep_groups<M, AXIS, V, K>(a, lds, blockIdx.x, a.groups, gridDim.x);
+	constexpr bool PF = K != store_f32 && AXIS != 0, NT = K == store_f32 && AXIS != 0;
+	// Workgroups are dispatched to the 8 XCDs in turn, so workgroups b and b + 8 run side by side on one XCD.
+	// Where a group's row segment on y/z is shorter than a 128-byte cache line (16-bit groups of fewer than 16
+	// lines), they take neighbouring groups, so the line's other part is found in that XCD's L2: measured on
+	// 400^3 x 8 BF16 with 8-line groups, DRAM reads of the y/z sweeps 2.05 -> 1.09 GB per launch (z 0.98 ->
+	// 0.88 ms).  Elsewhere the plain order is kept (reordered, the FP32 z sweep was 13 % slower).  The
+	// reordering is a permutation of the first W - W mod 8 workgroups; the rest keep their index.
+	constexpr int e = K == store_f32 ? 4 : 2;
+	const int64_t W = gridDim.x, W8 = W - W % 8, b = blockIdx.x;
+	const bool reorder = AXIS != 0 && (int64_t)a.lines_per_group * V * e < 128;
+	const int64_t first = reorder && b < W8 ? (b % 8) * (W8 / 8) + b / 8 : b;
+	if (a.base >= M - 1)
+		sweep_groups<M, AXIS, V, K, PF, NT, true>(a, lds, first, a.groups, W);
+	else
+		sweep_groups<M, AXIS, V, K, PF, NT, false>(a, lds, first, a.groups, W);
 }
 
 // Many microenvironments in one launch: workgroups walk a work list of (instance, group range) items.  The
@@ -1110,9 +1149,19 @@ __global__ __launch_bounds__(V == 1 ? batch_max_threads : 512) void sweep_batche
 		const batch_item w = items[it];
 		const part_args a = inst[w.instance];
 		if constexpr (V > 1 && AXIS != 0)
-			sweep_groups_wrapped<batch_chunk, AXIS, V, K>(a, lds, w.begin, w.end, 1);
+		{
+			if (a.base >= batch_chunk - 1)
+				sweep_groups_wrapped<batch_chunk, AXIS, V, K, true>(a, lds, w.begin, w.end, 1);
+			else
+				sweep_groups_wrapped<batch_chunk, AXIS, V, K, false>(a, lds, w.begin, w.end, 1);
+		}
 		else
-			sweep_groups<batch_chunk, AXIS, V, K, (K != store_f32), false>(a, lds, w.begin, w.end, 1);
+		{
+			if (a.base >= batch_chunk - 1)
+				sweep_groups<batch_chunk, AXIS, V, K, (K != store_f32), false, true>(a, lds, w.begin, w.end, 1);
+			else
+				sweep_groups<batch_chunk, AXIS, V, K, (K != store_f32), false, false>(a, lds, w.begin, w.end, 1);
+		}
 	}
 }
 

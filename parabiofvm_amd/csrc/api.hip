@@ -94,6 +94,10 @@ static void free_agents(bfvm_ctx* ctx)
 	dfree(a.win);
 	dfree(a.keys);
 	dfree(a.idx);
+	dfree(a.snum);
+	dfree(a.sden);
+	dfree(a.sfac);
+	dfree(a.multi);
 	if (a.sort_tmp)
 		BFVM_HIP(hipFree(a.sort_tmp));
 	a.sort_tmp = nullptr;
@@ -881,6 +885,10 @@ int bfvm_store_agents(bfvm_ctx* ctx, const bfvm_agents_desc* agents)
 		{
 			dalloc(a.keys, 2 * a.capacity);
 			dalloc(a.idx, 2 * a.capacity);
+			dalloc(a.snum, a.capacity * S);
+			dalloc(a.sden, a.capacity * S);
+			dalloc(a.sfac, a.capacity * S);
+			dalloc(a.multi, 1);
 		}
 		if (sorted != a.sorted)
 			ctx->epoch++;

@@ -353,6 +353,8 @@ struct agents_buffers
 	bool sorted = false;
 	uint64_t* keys = nullptr; // [2][capacity]
 	int* idx = nullptr;		  // [2][capacity]
+	float *snum = nullptr, *sden = nullptr, *sfac = nullptr; // the recompute terms in voxel order [capacity][S]
+	int* multi = nullptr; // set at upload when some voxel holds more than one agent
 	void* sort_tmp = nullptr;
 	size_t sort_tmp_bytes = 0;
 };

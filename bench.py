@@ -119,6 +119,19 @@ def profiled_traffic(kernel, storage, prefix="traffic_timestep_400cube_8sub"):
         return None, None
 
 
+def profiled_dram_traffic(kernel, storage, prefix="traffic_timestep_400cube_8sub"):
+    """DRAM bytes per launch of `kernel` read from the same summary's direct DRAM counters (reads + writes at the
+    L2's memory interface, no calibration); None when the summary has no such figure."""
+    path = os.path.join(ROOT, "profiles", f"{prefix}_{storage}.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        e = json.loads(block)[kernel]
+        return int(e["dram_read_bytes"]) + int(e["dram_write_bytes"])
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def run_batch(args, storage, cfgs=None):
     """64 independent microenvironments of uneven shapes and substrate counts advanced as one batch (one launch
     per sweep axis and one gradient launch over all instances, the step replayed as one HIP graph).  Returns
@@ -179,7 +192,9 @@ def run_batch(args, storage, cfgs=None):
         "kernel_gbs": {k: round(per[k] / (kernel_ms[k] * 1e-3) / 1e9, 1) for k in per},
         "roofline": {"bound": "hbm", "kernel": "batched " + dominant, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "traffic_source": source, "bytes_model": "step model (gradient 12 B/VS)",
+                     "traffic": traffic, "traffic_source": source,
+                     "traffic_dram": profiled_dram_traffic("batched_" + dominant, storage, prefix="traffic_batch64"),
+                     "bytes_model": "step model (gradient 12 B/VS)",
                      "algorithmic_bytes_per_launch": per[dominant], "avg_launch_ms": round(kernel_ms[dominant], 4)},
     }
     batch.close()
@@ -238,6 +253,8 @@ def bench_stress(args):
     def make(st):
         g = parabiofvm_amd.DeviceSolver(cfg["dims"], cfg["mins"], cfg["maxs"], cfg["voxel_shape"], S, cfg["dt"],
                                         cfg["D"], cfg["decay"], np.zeros(S, dtype=np.float32), device=0)
+        # heavy same-voxel contention: the sort-by-voxel reduction
+        g.set_secretion_reduction("sorted")
         apply_config(cfg, g)
         g.set_storage(st)
         g.initialize()
@@ -277,11 +294,11 @@ def bench_stress(args):
         kernel_ms[k] = lo.benchmark(k, args.kernel_iters)
         gbs = f", {per[k] / (kernel_ms[k] * 1e-3) / 1e9:.0f} GB/s" if k in per else ""
         log(f"[bench] {storage} {k}: {kernel_ms[k]:.4f} ms{gbs}")
-    # the sorted (segmented) reduction at the same contention, for comparison with the claim-and-atomics one
-    lo.set_secretion_reduction("sorted")
+    # the claim-and-atomics reduction at the same contention, for comparison with the sorted one
+    lo.set_secretion_reduction("atomic")
     lo.store_agents()
-    sorted_recompute = lo.benchmark("secretion_recompute", args.kernel_iters)
-    log(f"[bench] {storage} secretion_recompute with the sorted reduction: {sorted_recompute:.4f} ms")
+    atomic_recompute = lo.benchmark("secretion_recompute", args.kernel_iters)
+    log(f"[bench] {storage} secretion_recompute with the claim-and-atomics reduction: {atomic_recompute:.4f} ms")
     result = {
         "metric": METRIC, "value": VS * args.steps / elapsed, "unit": "voxel·substrate updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
@@ -293,7 +310,8 @@ def bench_stress(args):
         "hbm_frac_step": round((6 * e + 12) * VS / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
         "kernel_gbs": {k: round(per[k] / (kernel_ms[k] * 1e-3) / 1e9, 1) for k in per},
-        "secretion_recompute_sorted_ms": round(sorted_recompute, 4),
+        "secretion_reduction": "sorted",
+        "secretion_recompute_atomic_ms": round(atomic_recompute, 4),
         "error_vs_f32": {"steps": args.warmup + args.steps, "rel_l2": [float(v) for v in l2],
                          "max_rel": [float(v) for v in mx]},
         "roofline": None, "cpu_baseline": None,
@@ -413,6 +431,7 @@ def run(args):
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": source,
+                "traffic_dram": None if (args.shape or args.agents) else profiled_dram_traffic(dominant, args.storage),
                 "bytes_model": "step model: sweeps read + write, gradient its 3 FP32 component writes (12 B/VS)",
                 "algorithmic_bytes_per_launch": per_bytes[dominant], "avg_launch_ms": round(kernel_ms[dominant], 4)}
     if dominant == "gradient":

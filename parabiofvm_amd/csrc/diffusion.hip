@@ -440,42 +440,60 @@ struct part_args
 	int lines_per_group; // lines handled by one workgroup iteration
 	int64_t groups;
 	int vec; // substrates per thread (V)
+	// the two divisors of this axis's line decomposition (x: S/V and Y, y: X*S/V and S, z: S and X) as
+	// multiply-shift pairs: n / d = (n * mul) >> shift for every n < 2^31 (the host checks lines * V < 2^31)
+	uint32_t div_mul[2];
+	int div_shift[2];
 };
 
-// Line geometry with V substrates per line.  x: line = (row, substrate group); y: line = (z, q0/V) with
-// q0 = x*S + s0; z: line = q0/V over the whole (y, x, s) plane.
-template <int AXIS, int V>
-__device__ __forceinline__ void vline_coords(const sweep_geometry& g, int64_t line, int64_t& base, int& s0, int* coord)
+// n / d for n < 2^31 with (mul, shift) from fast_divisor(d): Granlund and Montgomery's multiply-shift division,
+// mul = ceil(2^(31+l) / d) with 2^l >= d, which fits 32 bits and is exact for 31-bit numerators
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t mul, int shift)
 {
+	return (uint32_t)(((uint64_t)n * mul) >> shift);
+}
+
+// Line geometry with V substrates per line.  x: line = (row, substrate group); y: line = (z, q0/V) with
+// q0 = x*S + s0; z: line = q0/V over the whole (y, x, s) plane.  Line indices and plane positions stay below
+// 2^31 (checked on the host), so the divisions are 32-bit multiply-shifts by precomputed divisors.
+template <int AXIS, int V>
+__device__ __forceinline__ void vline_coords(const part_args& a, int64_t line, int64_t& base, int& s0, int* coord)
+{
+	const sweep_geometry& g = a.g;
 	const int64_t XS = (int64_t)g.X * g.S;
+	const uint32_t l = (uint32_t)line;
 	if constexpr (AXIS == 0)
 	{
-		const int SG = g.S / V;
-		const int64_t row = line / SG;
-		s0 = (int)(line % SG) * V;
-		base = row * XS + s0;
+		const uint32_t SG = (uint32_t)(g.S / V);
+		const uint32_t row = fdiv(l, a.div_mul[0], a.div_shift[0]);
+		s0 = (int)(l - row * SG) * V;
+		base = (int64_t)row * XS + s0;
+		const uint32_t z = fdiv(row, a.div_mul[1], a.div_shift[1]);
 		coord[0] = 0;
-		coord[1] = (int)(row % g.Y);
-		coord[2] = (int)(row / g.Y);
+		coord[1] = (int)(row - z * (uint32_t)g.Y);
+		coord[2] = (int)z;
 	}
 	else if constexpr (AXIS == 1)
 	{
-		const int64_t XSV = XS / V;
-		const int64_t z = line / XSV;
-		const int64_t q0 = (line % XSV) * V;
-		s0 = (int)(q0 % g.S);
-		base = z * (int64_t)g.Y * XS + q0;
-		coord[0] = (int)(q0 / g.S);
+		const uint32_t XSV = (uint32_t)(XS / V);
+		const uint32_t z = fdiv(l, a.div_mul[0], a.div_shift[0]);
+		const uint32_t q0 = (l - z * XSV) * V;
+		const uint32_t x = fdiv(q0, a.div_mul[1], a.div_shift[1]);
+		s0 = (int)(q0 - x * (uint32_t)g.S);
+		base = (int64_t)z * g.Y * XS + q0;
+		coord[0] = (int)x;
 		coord[1] = 0;
 		coord[2] = (int)z;
 	}
 	else
 	{
-		const int64_t q0 = line * V;
-		s0 = (int)(q0 % g.S);
+		const uint32_t q0 = l * V;
+		const uint32_t xy = fdiv(q0, a.div_mul[0], a.div_shift[0]);
+		s0 = (int)(q0 - xy * (uint32_t)g.S);
 		base = q0;
-		coord[0] = (int)((q0 / g.S) % g.X);
-		coord[1] = (int)(q0 / XS);
+		const uint32_t y = fdiv(xy, a.div_mul[1], a.div_shift[1]);
+		coord[0] = (int)(xy - y * (uint32_t)g.X);
+		coord[1] = (int)y;
 		coord[2] = 0;
 	}
 }
@@ -658,7 +676,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 #pragma clang fp contract(fast)
 	using vec = typename vec_of<V>::type;
 	const sweep_geometry& g = a.g;
-	const int S = g.S;
+	int S = g.S; // redefined opaquely in every group iteration, below
 	const int P = a.P;
 	const int lpg = a.lines_per_group;
 	const int ncoef = 4 * a.ntypes * a.mb * S;
@@ -682,7 +700,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 
 	int line_local, k;
 	thread_slot<AXIS>(threadIdx.x, S / V, P, lpg, line_local, k);
-	const int64_t stride = line_stride<AXIS>(g);
+	int64_t stride = line_stride<AXIS>(g); // redefined opaquely in every group iteration, below
 	const int grid[3] = { g.X, g.Y, g.Z };
 	const size_t fstride = (size_t)a.ntypes * a.mb * S;
 	const uint32_t key = K == store_f32 ? 0u : rounding_key(g.rounding, 1 + AXIS);
@@ -712,7 +730,7 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 			int64_t b = 0;
 			int sx = 0;
 			int cd[3];
-			vline_coords<AXIS, V>(g, ln, b, sx, cd);
+			vline_coords<AXIS, V>(a, ln, b, sx, cd);
 			const int64_t o = b + (int64_t)start * stride;
 #pragma unroll
 			for (int j = 0; j < M; j++)
@@ -726,13 +744,19 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 
 	for (int64_t group = g_first; group < g_end; group += g_step)
 	{
+		// The unrolled element loops address memory at j * stride and the LDS tables at j * S for every j; held
+		// across the group loop those 16 + 16 uniform offsets overflow the scalar registers and spill into
+		// vector-register lanes (hundreds of SGPR spills per kernel).  Marking stride and S as redefined here
+		// makes them recomputed per group instead: measured on 640^3 x 16 BF16 (autotuned) sweeps 8.7 / 9.3 /
+		// 10.0 -> 7.5 / 8.7 / 8.5 ms, and 10-12 fewer VGPRs in every 4-substrate kernel.
+		asm volatile("" : "+s"(stride), "+s"(S));
 		const int64_t line = group * lpg + line_local;
 		const bool active = k >= 0 && line < a.lines;
 		int64_t base = 0;
 		int s0 = 0;
 		int coord[3] = { 0, 0, 0 };
 		if (active)
-			vline_coords<AXIS, V>(g, line, base, s0, coord);
+			vline_coords<AXIS, V>(a, line, base, s0, coord);
 		const int64_t off0 = base + (int64_t)start * stride; // element offset of the block's first value
 		const float* cr = lcoef + 0 * fstride + (size_t)type * a.mb * S + s0;
 		const float* cg = lcoef + 1 * fstride + (size_t)type * a.mb * S + s0;
@@ -841,6 +865,12 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 			// measured slower: the second copy of the loop costs registers)
 			const vec sc = vload<V>(lscale + s0);
 			uint32_t h = K == store_f32 ? 0u : sr_seed(off0, key);
+			// faces: one per-lane condition for the interior elements (a face of another axis through the line)
+			// and one each for the block's first and last element (this axis's faces at the line's ends); waves
+			// with no such lane skip the per-element checks
+			const bool face_lo = line_face || (start == 0 && end_min);
+			const bool face_hi = line_face || (start + mk == a.n && end_max);
+			const bool wave_face = __ballot(face_lo || face_hi) != 0;
 #pragma unroll
 			for (int j = 0; j < M; j++)
 			{
@@ -853,9 +883,12 @@ __device__ __forceinline__ void sweep_groups(const part_args& a, float* lds, int
 						x = u1;
 					else
 						x = v[j] - vload<V>(ca + j * S) * u0 - vload<V>(cgp + j * S) * u1;
-					const int i = start + j;
-					if (line_face || (i == 0 && end_min) || (i == a.n - 1 && end_max))
-						x = faces_slow<AXIS, V, false, K>(x, g, s0, coord, i, sc);
+					if (wave_face)
+					{
+						const bool fj = j == 0 ? (face_lo || (last(j) && face_hi)) : (last(j) ? face_hi : line_face);
+						if (fj)
+							x = faces_slow<AXIS, V, false, K>(x, g, s0, coord, start + j, sc);
+					}
 					// NT: FP32 y/z outputs of a large instance are streamed past the caches (measured: y 0.964 ->
 					// 0.947 ms, z 1.037 -> 1.023 ms on 400^3 x 8; along x, whose rows stay in cache, and for the
 					// small instances of a batch, whose outputs the next sweep finds in cache, they are slower)
@@ -881,7 +914,7 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 #pragma clang fp contract(fast)
 	using vec = typename vec_of<V>::type;
 	const sweep_geometry& g = a.g;
-	const int S = g.S;
+	int S = g.S; // redefined opaquely in every group iteration, below
 	const int P = a.P;
 	const int lpg = a.lines_per_group;
 	const int Sp = S + V - 1; // LDS row length of the per-substrate tables
@@ -918,7 +951,7 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 
 	int line_local, k;
 	thread_slot<AXIS>(threadIdx.x, 1, P, lpg, line_local, k);
-	const int64_t stride = line_stride<AXIS>(g);
+	int64_t stride = line_stride<AXIS>(g); // redefined opaquely in every group iteration, below
 	const int grid[3] = { g.X, g.Y, g.Z };
 	const size_t fstride = (size_t)a.ntypes * a.mb * Sp;
 	const uint32_t key = K == store_f32 ? 0u : rounding_key(g.rounding, 1 + AXIS);
@@ -944,7 +977,7 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 			int64_t b = 0;
 			int sx = 0;
 			int cd[3];
-			vline_coords<AXIS, V>(g, ln, b, sx, cd);
+			vline_coords<AXIS, V>(a, ln, b, sx, cd);
 			const int64_t o = b + (int64_t)start * stride;
 #pragma unroll
 			for (int j = 0; j < M; j++)
@@ -958,13 +991,19 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 
 	for (int64_t group = g_first; group < g_end; group += g_step)
 	{
+		// The unrolled element loops address memory at j * stride and the LDS tables at j * S for every j; held
+		// across the group loop those 16 + 16 uniform offsets overflow the scalar registers and spill into
+		// vector-register lanes (hundreds of SGPR spills per kernel).  Marking stride and S as redefined here
+		// makes them recomputed per group instead: measured on 640^3 x 16 BF16 (autotuned) sweeps 8.7 / 9.3 /
+		// 10.0 -> 7.5 / 8.7 / 8.5 ms, and 10-12 fewer VGPRs in every 4-substrate kernel.
+		asm volatile("" : "+s"(stride), "+s"(S));
 		const int64_t line = group * lpg + line_local;
 		const bool active = k >= 0 && line < a.lines;
 		int64_t base = 0;
 		int s0 = 0;
 		int coord[3] = { 0, 0, 0 };
 		if (active)
-			vline_coords<AXIS, V>(g, line, base, s0, coord);
+			vline_coords<AXIS, V>(a, line, base, s0, coord);
 		const int64_t off0 = base + (int64_t)start * stride; // element offset of the block's first value
 		const float* cr = lcoef + 0 * fstride + (size_t)type * a.mb * Sp + s0;
 		const float* cg = lcoef + 1 * fstride + (size_t)type * a.mb * Sp + s0;
@@ -1080,6 +1119,9 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 			}
 			const vec sc = cld(lscale + s0);
 			uint32_t h = K == store_f32 ? 0u : sr_seed(off0, key);
+			const bool face_lo = line_face || (start == 0 && end_min);
+			const bool face_hi = line_face || (start + mk == a.n && end_max);
+			const bool wave_face = __ballot(face_lo || face_hi) != 0;
 #pragma unroll
 			for (int j = 0; j < M; j++)
 			{
@@ -1092,9 +1134,12 @@ __device__ __forceinline__ void sweep_groups_wrapped(const part_args& a, float* 
 						x = u1;
 					else
 						x = v[j] - cld(ca + j * Sp) * u0 - cld(cgp + j * Sp) * u1;
-					const int i = start + j;
-					if (line_face || (i == 0 && end_min) || (i == a.n - 1 && end_max))
-						x = faces_slow<AXIS, V, true, K>(x, g, s0, coord, i, sc);
+					if (wave_face)
+					{
+						const bool fj = j == 0 ? (face_lo || (last(j) && face_hi)) : (last(j) ? face_hi : line_face);
+						if (fj)
+							x = faces_slow<AXIS, V, true, K>(x, g, s0, coord, start + j, sc);
+					}
 					dstore_sr_stream<K, V>(g.data, off0 + j * stride, x, h);
 				}
 			}
@@ -1688,8 +1733,18 @@ void launch_partition_coefficients(bfvm_ctx* ctx, int axis)
 #endif
 static constexpr int64_t sweep_max_blocks = BFVM_SWEEP_MAX_BLOCKS;
 
-static part_args make_part_args(const sweep_geometry& g, const partition& p, const float* coef, const float* red,
-								const float* A)
+// (mul, shift) of fdiv for divisor d (1 <= d < 2^31)
+static void fast_divisor(uint32_t d, uint32_t& mul, int& shift)
+{
+	int l = 0;
+	while ((1ull << l) < d)
+		l++;
+	mul = (uint32_t)(((1ull << (31 + l)) + d - 1) / d);
+	shift = 31 + l;
+}
+
+static part_args make_part_args(int axis, const sweep_geometry& g, const partition& p, const float* coef,
+								const float* red, const float* A)
 {
 	part_args a;
 	a.g = g;
@@ -1707,6 +1762,13 @@ static part_args make_part_args(const sweep_geometry& g, const partition& p, con
 	a.vec = p.vec;
 	a.lines_per_group = p.lpg;
 	a.groups = (a.lines + p.lpg - 1) / p.lpg;
+	if ((uint64_t)a.lines * p.vec >= (1ull << 31))
+		throw std::runtime_error("sweep: more than 2^31 line positions along one axis (mesh too large)");
+	const int64_t XS = (int64_t)g.X * g.S;
+	const int64_t d0 = axis == 0 ? g.S / p.vec : (axis == 1 ? XS / p.vec : g.S);
+	const int64_t d1 = axis == 0 ? g.Y : (axis == 1 ? g.S : g.X);
+	fast_divisor((uint32_t)d0, a.div_mul[0], a.div_shift[0]);
+	fast_divisor((uint32_t)d1, a.div_mul[1], a.div_shift[1]);
 	return a;
 }
 
@@ -1718,7 +1780,7 @@ template <int M, int AXIS, int V, int K>
 static void launch_partitioned_t(bfvm_ctx* ctx, const sweep_geometry& g, const axis_tables& t)
 {
 	const partition& p = t.part;
-	const part_args a = make_part_args(g, p, t.coef, t.red, t.A);
+	const part_args a = make_part_args(AXIS, g, p, t.coef, t.red, t.A);
 	const int threads = p.threads;
 	const size_t lds = partition_lds_bytes(p, g.S, p.lpg);
 	const int64_t cap = ctx->sweep_workgroups[AXIS] > 0 ? ctx->sweep_workgroups[AXIS] : sweep_max_blocks;
@@ -1800,7 +1862,7 @@ static tile_args make_tile_args(const bfvm_ctx* ctx, int axis, const sweep_geome
 								const float* coef, const float* red, const float* A)
 {
 	tile_args t;
-	t.a = make_part_args(g, p, coef, red, A);
+	t.a = make_part_args(axis, g, p, coef, red, A);
 	const size_t e = storage_bytes(ctx->storage);
 	const int VE = (int)(16 / e);
 	const int64_t XS = (int64_t)g.X * g.S;
@@ -2001,7 +2063,7 @@ void prepare_batch_sweep(bfvm_batch* b, int axis)
 		launch_partition_tables(ctx, axis, p, coef, red, A, b->stream);
 		sweep_geometry g = geometry(ctx);
 		g.rounding = b->rounding;
-		args[w][i] = make_part_args(g, p, coef, red, A);
+		args[w][i] = make_part_args(axis, g, p, coef, red, A);
 		auto& part = ax.parts[w];
 		part.lds = std::max(part.lds, partition_lds_bytes(p, S, p.lpg, V > 1 && axis != 0));
 		part.threads = std::max(part.threads, p.threads);

@@ -10,8 +10,9 @@ an MI355X).
     oracle (per-substrate relative L2 reported; FP16 <= 1e-3, BF16 <= its 1e-2 quantisation floor);
   * config 3 at full size (the 64-instance batch, edges 32..200, 1..16 substrates): 3 batched steps, every
     instance against the float oracle (FP32 storage, 1e-5) and against the double oracle (FP16 storage, 1e-3);
-  * config 4 at full size (640^3 x 16 = 4.2e9 densities, past 2^31 elements) through properties that do not need
-    a CPU run of the whole workload: diffusion-only mass per substrate equals the (1 + decay dt/3)^-3 closed form
+  * config 4 at full size (640^3 x 16 = 4.2e9 densities, past 2^31 elements), the whole stress workload (1e7
+    clustered agents, Dirichlet faces and interior voxels) in BF16 and FP16 storage against the double oracle after
+    3 steps, and through properties that do not need a CPU run of the whole workload: diffusion-only mass per substrate equals the (1 + decay dt/3)^-3 closed form
     (FP32 and BF16 storage), sampled x/y/z lines equal a double-precision tridiagonal solve of the same line
     (every line independent, so the sampled lines pin the sweep wherever they lie, including element offsets
     beyond 2^31), gradient planes equal the float oracle's on a 3-plane slab around them, and the voxel index of
@@ -441,3 +442,51 @@ def test_config4_agent_voxels(pkg):
     np.testing.assert_array_equal(got, exp)
     _, counts = np.unique((exp[:, 2].astype(np.int64) * 640 + exp[:, 1]) * 640 + exp[:, 0], return_counts=True)
     print(f"1e7 agents: most agents sharing one voxel {counts.max()}")
+
+
+@pytest.fixture(scope="module")
+def config4_double(oracle_mod):
+    """The stress workload (make_config(4): 640^3 x 16, 1e7 agents in 64 clusters, 6 Dirichlet faces + 0.5 %
+    interior voxels) and the double oracle (the reference's -DDOUBLE_PRECISION host path) after STEPS_FULL steps
+    without the gradient (the double gradient of 4.2e9 densities would take 100 GB); the oracle is kept alive
+    and its densities are a view onto its memory."""
+    from parabiofvm_amd.configs import apply_config, make_config
+
+    cfg = make_config(4)
+    o = oracle_mod.Oracle(*args_of(cfg), dtype=np.float64)
+    apply_config(cfg, o)
+    o.initialize()
+    for i in range(STEPS_FULL):
+        o.timestep(i % 10 == 0, with_gradient=False)
+    yield cfg, o
+    del o
+
+
+@pytest.mark.timeout(1200)
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_config4_full_size_vs_double(storage, config4_double, pkg):
+    """The stress workload in 16-bit storage against the double oracle of the same workload after STEPS_FULL
+    steps (the sort-by-voxel reduction, as the stress bench runs it): per-substrate relative L2 and largest
+    relative error are reported; FP16 (scaled) meets 1e-3, BF16 its 1e-2 quantisation floor."""
+    cfg, o = config4_double
+    from parabiofvm_amd.configs import apply_config
+
+    g = pkg.DeviceSolver(*args_of(cfg))
+    g.set_secretion_reduction("sorted")
+    apply_config(cfg, g)
+    g.set_storage(storage)
+    g.initialize()
+    g.store()
+    for i in range(STEPS_FULL):
+        g.timestep(i % 10 == 0, True)
+    g.load_densities()
+    g.close()
+    ref = o.densities
+    l2 = rel_l2_by_substrate(g.densities, ref, cfg["S"])
+    worst, at = max_rel(g.densities, ref, tiny=1e-3)
+    l2s = " ".join(f"{v:.1e}" for v in l2)
+    print(f"config 4, {storage} storage, {STEPS_FULL} steps vs the double oracle: relative L2 per substrate "
+          f"[{l2s}] (max {l2.max():.2e}); largest relative error (|ref| floored at 1e-3) {worst:.2e}", flush=True)
+    assert np.all(np.isfinite(g.densities[:CHUNK]))
+    bound = 1e-3 if storage == "f16" else 1e-2
+    assert l2.max() <= bound, l2

@@ -337,18 +337,31 @@ __device__ __forceinline__ void gradient_tile(const gradient_args& a, int tile)
 				break;
 			const int y = y0 + r;
 			const int64_t orow = o + (int64_t)r * XS;
-			// x neighbours: the same substrates of voxel x +- 1 sit SG lanes away in this wave's registers;
-			// only the lanes at the wave's edges load theirs
-			vec xp = shfl_vec<V>(c[r], lane + SG);
-			vec xm = shfl_vec<V>(c[r], lane - SG);
-			if (xhi)
-				xp = vec(0.f);
-			else if (lane + SG >= wave_lanes)
-				xp = ld(orow + S);
-			if (xlo)
-				xm = vec(0.f);
-			else if (lane < SG)
-				xm = ld(orow - S);
+			// x neighbours.  One element per thread: loaded, with the address clamped to the thread's own
+			// element at the row's ends (the loads hit the neighbouring lanes' lines in cache; measured on the
+			// 64-instance batch, whose one-element instances hold 70 % of its densities: gradient FP32 4.16 ->
+			// 3.89 ms, FP16 3.91 -> 3.77 ms, against lane shuffles with edge loads under divergent branches).
+			// Vectors: the same substrates of voxel x +- 1 sit SG lanes away in this wave's registers; only
+			// the lanes at the wave's edges load theirs.
+			vec xp, xm;
+			if constexpr (V == 1)
+			{
+				xp = ld(xhi ? orow : orow + S);
+				xm = ld(xlo ? orow : orow - S);
+			}
+			else
+			{
+				xp = shfl_vec<V>(c[r], lane + SG);
+				xm = shfl_vec<V>(c[r], lane - SG);
+				if (xhi)
+					xp = vec(0.f);
+				else if (lane + SG >= wave_lanes)
+					xp = ld(orow + S);
+				if (xlo)
+					xm = vec(0.f);
+				else if (lane < SG)
+					xm = ld(orow - S);
+			}
 			const vec ym = r == 0 ? below : c[r - 1 < 0 ? 0 : r - 1];
 			const vec yp = r == ny - 1 ? above : c[r + 1 < TY ? r + 1 : TY - 1];
 			vec gx, gy, gz;

@@ -63,6 +63,33 @@ def run_batch(pkg, oracle_mod, cfgs, steps, storage="f32"):
     return batch, oracles, solvers
 
 
+def numpy_gradient(dens, grid, S, vs):
+    """Central differences of a 3-D field in float32, one-sided at the mesh ends, in the reference's expression
+    order (host/gradient_solver.cpp): the gradient of the densities as stored, [d][z][y][x][s]."""
+    X, Y, Z = grid
+    D = dens.reshape(Z, Y, X, S)
+    out = np.empty((3, Z, Y, X, S), dtype=np.float32)
+    for d, (axis, n, h) in enumerate(((2, X, vs[0]), (1, Y, vs[1]), (0, Z, vs[2]))):
+        b = np.float32(1) / np.float32(h)
+        c = np.float32(1) / np.float32(h * 2)
+        A = np.moveaxis(D, axis, 0)
+        G = np.moveaxis(out[d], axis, 0)
+        G[1:n - 1] = (A[2:] - A[:n - 2]) * c
+        G[0] = (A[1] - A[0]) * b
+        G[n - 1] = (A[n - 1] - A[n - 2]) * b
+    return out.reshape(-1)
+
+
+def check_gradients_of_stored(cfgs, solvers, tag=""):
+    """The batched gradient of 16-bit densities equals the gradient of those densities as loaded (the same FP32
+    values the kernel computes with)."""
+    for k, (cfg, g) in enumerate(zip(cfgs, solvers)):
+        exp = numpy_gradient(g.densities, cfg["grid"], cfg["S"], cfg["voxel_shape"])
+        scale = max(float(np.abs(g.densities).max()), 1e-6) / min(cfg["voxel_shape"])
+        err = np.abs(g.gradients.astype(np.float64) - exp) / scale
+        assert err.max() <= 1e-6, f"{tag} instance {k} grid {cfg['grid']} S {cfg['S']}: gradient {err.max():.2e}"
+
+
 def check_instances(cfgs, oracles, solvers, steps):
     for k, (cfg, o, g) in enumerate(zip(cfgs, oracles, solvers)):
         tag = f"instance {k} grid {cfg['grid']} S {cfg['S']}"
@@ -154,6 +181,7 @@ def test_batch_16bit_storage(storage, pkg, oracle_mod):
         d = g.densities.astype(np.float64) - o.densities
         rel = np.linalg.norm(d) / max(np.linalg.norm(o.densities.astype(np.float64)), 1e-30)
         assert rel <= bound, f"instance {k} grid {cfg['grid']} S {cfg['S']}: rel L2 {rel:.2e}"
+    check_gradients_of_stored(cfgs, solvers, storage)
     batch.close()
 
 
@@ -199,6 +227,7 @@ def test_batch_vector_widths_with_dirichlet_16bit(storage, pkg, oracle_mod):
         d = g.densities.astype(np.float64) - o.densities
         rel = np.linalg.norm(d) / max(np.linalg.norm(o.densities.astype(np.float64)), 1e-30)
         assert rel <= bound, f"instance {k} grid {cfg['grid']} S {cfg['S']}: rel L2 {rel:.2e}"
+    check_gradients_of_stored(cfgs, solvers, storage)
     batch.close()
 
 

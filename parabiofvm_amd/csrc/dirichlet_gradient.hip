@@ -399,12 +399,11 @@ __device__ __forceinline__ void gradient_tile(const gradient_args& a, int tile)
 	}
 }
 
-// One-element instances (S % 4 != 0) of FP32 batches in vectors of 4 consecutive row positions q..q+3
-// (q = x S + s): each lane moves 16 bytes per access instead of 4, with a quarter of the memory instructions.
-// Measured on the 64-instance batch (its one-element instances hold 70 % of its densities): gradient FP32
-// 3.89 -> 3.09 ms.  16-bit storage keeps one element per thread: its 8-byte vectors at 2-byte alignment split
-// into 2-byte loads, and without the one-plane prefetch of the one-element form the FP16 gradient ran 3.77 ->
-// 4.11 ms.  The
+// One-element instances (S % 4 != 0) of batches in vectors of 4 consecutive row positions q..q+3
+// (q = x S + s): each lane moves 16 (FP32) or 8 bytes per access instead of 4 or 2, with a quarter of the
+// memory instructions.  Measured on the 64-instance batch (its one-element instances hold 70 % of its
+// densities): gradient FP32 3.89 -> 3.09 ms, FP16 3.77 -> 3.28 ms (16-bit with the one-plane prefetch; without
+// it 4.11 ms).  The
 // positions of a vector belong to different substrates and voxels: rows need not start on a vector boundary,
 // so accesses are unaligned vectors; the x neighbours (q +- S) are unaligned vector loads (cache hits: the
 // neighbouring lanes' own row); the x boundary is decided per component (x = 0 where q + c < S, x = X - 1 where
@@ -503,15 +502,60 @@ __device__ __forceinline__ void gradient_tile_flat4(const gradient_args& a, int 
 		zm[r] = (r < ny && z0 > 0) ? ld((int64_t)(z0 - 1) * plane + (int64_t)r * XS) : vec(0.f);
 		cc[r] = r < ny ? ld((int64_t)z0 * plane + (int64_t)r * XS) : vec(0.f);
 	}
+	// 16-bit: plane z + 1 and the y neighbours of plane z are loaded (raw) one plane step ahead
+	constexpr bool prefetch = K != store_f32;
+	using raw = typename svec_of<K, 4>::type;
+	raw pzp[prefetch ? TY : 1], pbelow, pabove;
+	auto rld = [&](int64_t o) -> raw {
+		raw r;
+		if (whole)
+			r = gload((const uv*)(dens + o));
+		else
+		{
+#pragma unroll
+			for (int c = 0; c < 4; c++)
+				r[c] = q + c < XS ? gload(dens + o + c) : (elem)0.f;
+		}
+		return r;
+	};
+	auto cvt = [&](raw r) -> vec { return __builtin_convertvector(r, vec) * is; };
+	auto fetch = [&](int zz) {
+		const int64_t oo = (int64_t)zz * plane;
+#pragma unroll
+		for (int r = 0; r < TY; r++)
+			if (r < ny && zz + 1 < Z)
+				pzp[r] = rld(oo + plane + (int64_t)r * XS);
+		if (y0 > 0)
+			pbelow = rld(oo - XS);
+		if (y0 + ny < Y)
+			pabove = rld(oo + (int64_t)ny * XS);
+	};
+	if constexpr (prefetch)
+		if (z0 < z1)
+			fetch(z0);
 	for (int z = z0; z < z1; z++)
 	{
 		const int64_t o = (int64_t)z * plane;
 		vec zp[TY];
+		vec below, above;
+		if constexpr (prefetch)
+		{
 #pragma unroll
-		for (int r = 0; r < TY; r++)
-			zp[r] = (r < ny && z + 1 < Z) ? ld(o + plane + (int64_t)r * XS) : vec(0.f);
-		const vec below = y0 > 0 ? ld(o - XS) : vec(0.f);
-		const vec above = y0 + ny < Y ? ld(o + (int64_t)ny * XS) : vec(0.f);
+			for (int r = 0; r < TY; r++)
+				zp[r] = (r < ny && z + 1 < Z) ? cvt(pzp[r]) : vec(0.f);
+			below = y0 > 0 ? cvt(pbelow) : vec(0.f);
+			above = y0 + ny < Y ? cvt(pabove) : vec(0.f);
+			if (z + 1 < z1)
+				fetch(z + 1);
+		}
+		else
+		{
+#pragma unroll
+			for (int r = 0; r < TY; r++)
+				zp[r] = (r < ny && z + 1 < Z) ? ld(o + plane + (int64_t)r * XS) : vec(0.f);
+			below = y0 > 0 ? ld(o - XS) : vec(0.f);
+			above = y0 + ny < Y ? ld(o + (int64_t)ny * XS) : vec(0.f);
+		}
 #pragma unroll
 		for (int r = 0; r < TY; r++)
 		{
@@ -742,11 +786,8 @@ __global__ __launch_bounds__(grad_tile) void gradient_batched_kernel(const gradi
 					if constexpr (V == 0)
 						gradient_tile<4, K, TY>(a, (int)t);
 				}
-				else if (K == store_f32 && a.vec == flat4_vec)
-				{
-					if constexpr (K == store_f32)
-						gradient_tile_flat4<K, TY>(a, (int)t);
-				}
+				else if (a.vec == flat4_vec)
+					gradient_tile_flat4<K, TY>(a, (int)t);
 				else
 					gradient_tile<1, K, TY>(a, (int)t);
 			}
@@ -843,7 +884,7 @@ void prepare_batch_gradient(bfvm_batch* b)
 		// tiles of `zc` z planes: ask for (row tiles) x ceil(Z / zc)
 		const int rows = b->tuning.gradient_rows, zc = std::max(1, b->tuning.gradient_zchunk);
 		const bool stacked = b->tuning.gradient_stacked != 0;
-		const bool flat = ctx->storage == BFVM_STORAGE_F32;
+		const bool flat = true; // one-element instances as 4 row positions per thread
 		const gradient_args probe = make_gradient_args(ctx, 1, rows, stacked, flat);
 		const int64_t row_tiles = (int64_t)probe.ntiles;
 		args[i] = make_gradient_args(ctx, row_tiles * ((ctx->mesh.grid[2] + zc - 1) / zc), rows, stacked, flat);

@@ -221,13 +221,14 @@ struct gradient_args
 };
 
 // Rows of y per thread in the 3-D gradient (TY): the z window of every row stays in registers and the rows
-// of a thread share their y neighbours, so one plane step loads TY + 2 rows for TY outputs.  Measured on
-// 400^3 x 8: 8 rows for FP32 with 4 substrates per thread (1.89 ms against 2.42 with one row), 4 rows
-// otherwise (16-bit storage and the batched form are slower with 8).
+// of a thread share their y neighbours, so one plane step loads TY + 2 rows for TY outputs.  4 rows: on
+// 400^3 x 8 FP32 the tuner's best shape runs 1.72 ms with 4 rows (stacked tiles, whose 4 waves share their
+// halo rows through LDS) against 1.78 ms with 8 (side-by-side tiles; 8 rows had been 1.89 ms against 2.42 with
+// one row before the z-chunked tiles); 16-bit storage and the batched form are slower with 8.
 template <int V, int K>
 constexpr int gradient_rows()
 {
-	return (K == store_f32 && V == 4) ? 8 : 4;
+	return 4;
 }
 
 constexpr int wave_lanes = 64;

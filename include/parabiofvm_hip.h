@@ -194,12 +194,17 @@ int bfvm_set_gradient_tuning(bfvm_ctx* ctx, int32_t workgroups);
 /* Tile layout of the 3-D gradient: 0 = 256 columns by TY rows per workgroup, 1 = the workgroup's four waves
  * stacked along y (64 columns by 4 TY rows, the rows where waves meet passed through LDS).  Same results. */
 int bfvm_set_gradient_stacked(bfvm_ctx* ctx, int32_t stacked);
+/* Workgroup order of the 3-D gradient: 0 = each XCD runs consecutive z chunks of one row band, 1 = plane
+ * order (the workgroups resident together cover one z chunk of every row band).  Same results.  No
+ * counterpart in the reference, whose gradient is a host loop (host/gradient_solver.cpp:275-291). */
+int bfvm_set_gradient_order(bfvm_ctx* ctx, int32_t order);
 
 /* Autotuner, in place of the reference's fixed device launch shapes (work_block_size = 30,
  * CUDA/diffusion_solver.cpp:11; 256-thread blocks, CUDA/diffusion_solver.cu:947-987): times candidate shapes
  * of every sweep axis (register chunk, lines or rows per workgroup, workgroup count) and of the 3-D gradient
- * (workgroups, tile layout) on this context's mesh and keeps the fastest, as bfvm_set_sweep_tuning /
- * bfvm_set_sweep_workgroups / bfvm_set_gradient_tuning / bfvm_set_gradient_stacked would set them.  Results are cached per (axis, mesh
+ * (workgroups, tile layout, workgroup order) on this context's mesh and keeps the fastest, as
+ * bfvm_set_sweep_tuning / bfvm_set_sweep_workgroups / bfvm_set_gradient_tuning / bfvm_set_gradient_stacked /
+ * bfvm_set_gradient_order would set them.  Results are cached per (axis, mesh
  * shape, substrate count, storage, algorithm) for the process, so a second context of the same shape tunes
  * at no cost.  The candidates run on a device copy of the densities: the densities are unchanged and the
  * gradient buffer is recomputed from them.  Call after bfvm_initialize (and after bfvm_set_storage /

@@ -112,6 +112,7 @@ PROTOTYPES = {
     "bfvm_set_gradient_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "bfvm_set_sweep_workgroups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "bfvm_set_gradient_stacked": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "bfvm_set_gradient_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "bfvm_autotune": (ctypes.c_int, [ctypes.c_void_p]),
     "bfvm_batch_autotune": (ctypes.c_int, [ctypes.c_void_p]),
     "bfvm_tuning_save": (ctypes.c_int, [ctypes.c_char_p]),

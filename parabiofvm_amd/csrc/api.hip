@@ -699,6 +699,19 @@ int bfvm_set_gradient_stacked(bfvm_ctx* ctx, int32_t stacked)
 	});
 }
 
+int bfvm_set_gradient_order(bfvm_ctx* ctx, int32_t order)
+{
+	return guarded("bfvm_set_gradient_order", [&] {
+		if (!ctx)
+			throw std::invalid_argument("null context");
+		if (order != 0 && order != 1)
+			throw std::invalid_argument("gradient order must be 0 or 1");
+		ctx->gradient_order = order;
+		ctx->epoch++;
+		return BFVM_OK;
+	});
+}
+
 int bfvm_autotune(bfvm_ctx* ctx)
 {
 	return guarded("bfvm_autotune", [&] {

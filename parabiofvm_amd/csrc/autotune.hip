@@ -33,6 +33,7 @@ struct gradient_shape
 {
 	int groups = 0;
 	bool stacked = false;
+	int order = 0;
 };
 std::map<shape_key, gradient_shape> g_gradient_cache;
 std::map<std::string, bfvm_batch::tuning_t> g_batch_cache; // key: storage and every instance's shape
@@ -122,7 +123,7 @@ sweep_shape search_sweep(bfvm_ctx* ctx, int axis, int reps)
 		}
 	if (best_t >= 1e30f)
 		throw std::runtime_error("autotune: no sweep shape fits axis " + std::to_string(axis));
-	for (int w : { 1024, 2048, 4096, 16384 })
+	for (int w : { 1024, 2048, 4096, 16384, 65536 })
 	{
 		sweep_shape cand = best;
 		cand.workgroups = w;
@@ -161,7 +162,7 @@ bool apply_cached_batch_tuning(bfvm_batch* b)
 
 // Text file, one entry per line:
 //   sweep <axis> <X> <Y> <Z> <S> <storage> <algorithm> <chunk> <lines> <workgroups>
-//   gradient <X> <Y> <Z> <S> <storage> <algorithm> <workgroups> <stacked>
+//   gradient <X> <Y> <Z> <S> <storage> <algorithm> <workgroups> <stacked> [<order>]
 //   batch <key> <group threads [3][2]> <max blocks [3][2]> <gradient tiles> <gradient blocks> <gradient rows>
 //         <gradient z planes> <gradient split> <gradient stacked>
 void save_tuning(const char* path)
@@ -174,8 +175,8 @@ void save_tuning(const char* path)
 		std::fprintf(f, "sweep %d %d %d %d %d %d %d %d %d %d\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
 					 std::get<3>(k), std::get<4>(k), std::get<5>(k), std::get<6>(k), v.chunk, v.lines, v.workgroups);
 	for (const auto& [k, v] : g_gradient_cache)
-		std::fprintf(f, "gradient %d %d %d %d %d %d %d %d\n", std::get<1>(k), std::get<2>(k), std::get<3>(k),
-					 std::get<4>(k), std::get<5>(k), std::get<6>(k), v.groups, v.stacked ? 1 : 0);
+		std::fprintf(f, "gradient %d %d %d %d %d %d %d %d %d\n", std::get<1>(k), std::get<2>(k), std::get<3>(k),
+					 std::get<4>(k), std::get<5>(k), std::get<6>(k), v.groups, v.stacked ? 1 : 0, v.order);
 	for (const auto& [k, t] : g_batch_cache)
 	{
 		std::fprintf(f, "batch %s", k.c_str());
@@ -228,10 +229,13 @@ void load_tuning(const char* path)
 		}
 		else if (kind == "gradient")
 		{
-			int X, Y, Z, S, st, alg, groups, stacked;
+			int X, Y, Z, S, st, alg, groups, stacked, order = 0;
 			ok = static_cast<bool>(in >> X >> Y >> Z >> S >> st >> alg >> groups >> stacked);
+			if (ok && !(in >> order)) // files written before the order field hold 8 numbers
+				order = 0;
+			ok = ok && (order == 0 || order == 1);
 			if (ok)
-				g_gradient_cache[shape_key { -1, X, Y, Z, S, st, alg }] = gradient_shape { groups, stacked != 0 };
+				g_gradient_cache[shape_key { -1, X, Y, Z, S, st, alg }] = gradient_shape { groups, stacked != 0, order };
 		}
 		else if (kind == "batch")
 		{
@@ -300,23 +304,26 @@ void autotune_context(bfvm_ctx* ctx, bool tune_gradient)
 		if (!cached)
 		{
 			float best_t = 1e30f;
-			for (bool stacked : { false, true })
-				for (int g : { 2048, 4096, 8192, 16384, 32768, 65536 })
-				{
-					ctx->gradient_groups = g;
-					ctx->gradient_stacked = stacked;
-					const float t = best_ms(ctx->stream, reps, [&] { launch_gradient(ctx); });
-					if (t < best_t)
+			for (int order : { 0, 1 })
+				for (bool stacked : { false, true })
+					for (int g : { 2048, 4096, 8192, 16384, 32768, 65536 })
 					{
-						best_t = t;
-						best = gradient_shape { g, stacked };
+						ctx->gradient_groups = g;
+						ctx->gradient_stacked = stacked;
+						ctx->gradient_order = order;
+						const float t = best_ms(ctx->stream, reps, [&] { launch_gradient(ctx); });
+						if (t < best_t)
+						{
+							best_t = t;
+							best = gradient_shape { g, stacked, order };
+						}
 					}
-				}
 			std::lock_guard<std::mutex> lock(g_cache_mutex);
 			g_gradient_cache[key] = best;
 		}
 		ctx->gradient_groups = best.groups;
 		ctx->gradient_stacked = best.stacked;
+		ctx->gradient_order = best.order;
 	}
 	BFVM_HIP(hipStreamSynchronize(ctx->stream));
 	ctx->epoch++;
@@ -343,7 +350,7 @@ void autotune_batch(bfvm_batch* b)
 			{
 				if (threads > max_threads)
 					continue;
-				for (int blocks : { 1024, 2048, 4096 })
+				for (int blocks : { 1024, 2048, 4096, 8192 })
 				{
 					b->tuning.group_threads[axis][w] = threads;
 					b->tuning.max_blocks[axis][w] = blocks;

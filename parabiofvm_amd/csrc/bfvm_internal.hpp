@@ -460,6 +460,7 @@ struct bfvm_ctx
 	int lines_request[3] = { 0, 0, 0 };
 	int gradient_groups = 0; // workgroups the 3-D gradient aims for (0 = automatic)
 	bool gradient_stacked = false; // 3-D gradient tiles with the workgroup's waves stacked along y
+	int gradient_order = 0; // 3-D gradient workgroup order: 0 = z chunks of a row band per XCD, 1 = plane order
 	int sweep_workgroups[3] = { 0, 0, 0 }; // most workgroups of a sweep launch per axis (0 = the built-in cap)
 
 	bfvm::agents_buffers ag;

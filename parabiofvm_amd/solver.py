@@ -236,6 +236,10 @@ class DeviceSolver:
         """3-D gradient tiles with the workgroup's waves stacked along y (True) or side by side (False)."""
         check("bfvm_set_gradient_stacked", self.lib.bfvm_set_gradient_stacked(self.h, int(bool(stacked))))
 
+    def set_gradient_order(self, order):
+        """3-D gradient workgroup order: 0 = z chunks of a row band per XCD, 1 = plane order."""
+        check("bfvm_set_gradient_order", self.lib.bfvm_set_gradient_order(self.h, int(order)))
+
     def set_gradient_tuning(self, workgroups):
         check("bfvm_set_gradient_tuning", self.lib.bfvm_set_gradient_tuning(self.h, int(workgroups)))
 

@@ -60,22 +60,27 @@ def test_cpp_facade_built_and_linked():
 
 def test_tuning_cache_file_round_trip(tmp_path):
     """The launch-shape tuning cache file (bfvm_tuning_save / bfvm_tuning_load): entries written by hand are read
-    back unchanged, and a malformed line is an argument error with the line number (no device work)."""
+    back unchanged (a gradient entry without the workgroup order, as files from before that field hold, reads as
+    order 0), and a malformed line is an argument error with the line number (no device work)."""
     from parabiofvm_amd import _native
 
     src = tmp_path / "in.txt"
     src.write_text("sweep 2 400 400 400 8 0 0 16 16 4096\ngradient 400 400 400 8 0 0 8192 1\n"
+                   "gradient 200 100 50 4 2 0 4096 0 1\n"
                    "batch 0:1;32,40,50,3,3 256 128 256 256 512 256 2048 2048 1024 2048 4096 2048 4 8192 8 32 1 1\n")
     _native.tuning_load(str(src))
     out = tmp_path / "out.txt"
     _native.tuning_save(str(out))
     lines = set(out.read_text().splitlines())
     assert "sweep 2 400 400 400 8 0 0 16 16 4096" in lines
-    assert "gradient 400 400 400 8 0 0 8192 1" in lines
+    assert "gradient 400 400 400 8 0 0 8192 1 0" in lines
+    assert "gradient 200 100 50 4 2 0 4096 0 1" in lines
     assert "batch 0:1;32,40,50,3,3 256 128 256 256 512 256 2048 2048 1024 2048 4096 2048 4 8192 8 32 1 1" in lines
     bad = tmp_path / "bad.txt"
     bad.write_text("sweep 2 400\n")
     lib = _native.load()
     assert lib.bfvm_tuning_load(str(bad).encode()) == -1
     assert b"bad.txt:1" in lib.bfvm_last_error()
+    bad.write_text("gradient 400 400 400 8 0 0 8192 1 5\n")
+    assert lib.bfvm_tuning_load(str(bad).encode()) == -1
     assert lib.bfvm_tuning_load(str(tmp_path / "missing.txt").encode()) == -1

@@ -7,6 +7,8 @@ Tolerances (north-star contract for the FP32 path):
   * secretion/uptake: <= 1e-5 relative (float atomics sum in a different order), voxel indices exact;
   * golden values: within 1e-5 relative + 4 float ULPs of the reference's printed expectation.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -258,7 +260,8 @@ def test_gpu_gradient_stacked_tiles(shape, S, storage, pkg, oracle_mod):
     """The gradient's stacked tile layout (waves stacked along y, the rows where they meet passed through LDS) on
     meshes whose rows and columns do not fill whole tiles: bit-identical to the float oracle (FP32 storage) and
     to the other tile layout (FP16 storage), at several workgroup targets (z split into chunks of different
-    lengths, so tiles start and end at every plane parity)."""
+    lengths, so tiles start and end at every plane parity), in both workgroup orders (the plane order covers
+    every tile exactly once: the output is NaN-filled before each launch)."""
     p = random_problem(3, shape, S, seed=45)
     o, g = build_pair(pkg, oracle_mod, p)
     if storage != "f32":
@@ -266,19 +269,28 @@ def test_gpu_gradient_stacked_tiles(shape, S, storage, pkg, oracle_mod):
         g.initialize()
     g.store()
     o.gradient()
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.restype = ctypes.c_int
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    poison = np.full(g.gradients.size, np.nan, dtype=np.float32)
     results = []
-    for stacked in (False, True):
-        for groups in (1, 3, 64, 4096):
-            g.set_gradient_stacked(stacked)
-            g.set_gradient_tuning(groups)
-            g.gradients[:] = np.nan
-            g.gradient()
-            g.load()
-            if storage == "f32":
-                assert_bitwise(g.gradients, o.gradients, f"stacked {stacked} groups {groups}")
-            results.append(g.gradients.copy())
+    for order in (0, 1):
+        for stacked in (False, True):
+            for groups in (1, 3, 64, 4096):
+                g.set_gradient_order(order)
+                g.set_gradient_stacked(stacked)
+                g.set_gradient_tuning(groups)
+                # NaN in the device gradient buffer (the context is idle after load), so a tile left out shows
+                assert hip.hipMemcpy(g.device_buffers().gradients, poison.ctypes.data, poison.nbytes, 1) == 0
+                g.gradient()
+                g.load()
+                if storage == "f32":
+                    assert_bitwise(g.gradients, o.gradients, f"order {order} stacked {stacked} groups {groups}")
+                results.append(g.gradients.copy())
     for r in results[1:]:
-        assert_bitwise(r, results[0], "tile layouts and z chunkings")
+        assert_bitwise(r, results[0], "tile layouts, workgroup orders and z chunkings")
+    with pytest.raises(RuntimeError):
+        g.set_gradient_order(2)
 
 
 @pytest.mark.parametrize("name,dims,shape,S", SHAPES, ids=[s[0] for s in SHAPES])

@@ -656,6 +656,29 @@ __device__ __forceinline__ void gradient_tile_stacked(const gradient_args& a, in
 		zm[r] = (run && r < ny && z0 > 0) ? ld((int64_t)(z0 - 1) * plane + (int64_t)r * XS) : vec(0.f);
 		c[r] = (run && r < ny) ? ld((int64_t)z0 * plane + (int64_t)r * XS) : vec(0.f);
 	}
+	// 16-bit forms load plane z + 1 and the halo rows of plane z (raw) one plane step ahead, as the side-by-side
+	// tile does.  Loaded and converted in the same step, each conditional 16-bit load was waited for where it
+	// was issued (the conversion sits inside its branch), so the step's loads ran one after another: 400^3 x 8
+	// FP16 stacked tiles 2.05-2.42 ms against 1.46-1.75 ms for FP32.
+	constexpr bool prefetch = K != store_f32;
+	using raw = typename svec_of<K, V>::type;
+	raw pzp[prefetch ? TY : 1], pbelow, pabove;
+	const bool need_below = run && !below_wave && y0 > 0;
+	const bool need_above = run && !above_wave && y0 + ny < Y;
+	auto fetch = [&](int zz) {
+		const int64_t oo = (int64_t)zz * plane;
+#pragma unroll
+		for (int r = 0; r < TY; r++)
+			if (run && r < ny && zz + 1 < Z)
+				pzp[r] = draw<K, V>(dens, off + oo + plane + (int64_t)r * XS);
+		if (need_below)
+			pbelow = draw<K, V>(dens, off + oo - XS);
+		if (need_above)
+			pabove = draw<K, V>(dens, off + oo + (int64_t)ny * XS);
+	};
+	if constexpr (prefetch)
+		if (z0 < z1)
+			fetch(z0);
 	for (int z = z0; z < z1; z++)
 	{
 		const int64_t o = (int64_t)z * plane;
@@ -664,14 +687,29 @@ __device__ __forceinline__ void gradient_tile_stacked(const gradient_args& a, in
 		buf[(2 * wave) * wave_lanes + lane] = c[0];
 		buf[(2 * wave + 1) * wave_lanes + lane] = c[ny > 0 ? ny - 1 : 0];
 		vec zp[TY];
-#pragma unroll
-		for (int r = 0; r < TY; r++)
-			zp[r] = (run && r < ny && z + 1 < Z) ? ld(o + plane + (int64_t)r * XS) : vec(0.f);
 		vec below = vec(0.f), above = vec(0.f);
-		if (run && !below_wave && y0 > 0)
-			below = ld(o - XS);
-		if (run && !above_wave && y0 + ny < Y)
-			above = ld(o + (int64_t)ny * XS);
+		if constexpr (prefetch)
+		{
+#pragma unroll
+			for (int r = 0; r < TY; r++)
+				zp[r] = (run && r < ny && z + 1 < Z) ? dcvt<K, V>(pzp[r]) * is : vec(0.f);
+			if (need_below)
+				below = dcvt<K, V>(pbelow) * is;
+			if (need_above)
+				above = dcvt<K, V>(pabove) * is;
+			if (z + 1 < z1)
+				fetch(z + 1);
+		}
+		else
+		{
+#pragma unroll
+			for (int r = 0; r < TY; r++)
+				zp[r] = (run && r < ny && z + 1 < Z) ? ld(o + plane + (int64_t)r * XS) : vec(0.f);
+			if (need_below)
+				below = ld(o - XS);
+			if (need_above)
+				above = ld(o + (int64_t)ny * XS);
+		}
 		__syncthreads();
 		if (run)
 		{
@@ -690,14 +728,35 @@ __device__ __forceinline__ void gradient_tile_stacked(const gradient_args& a, in
 				continue;
 			const int y = y0 + r;
 			const int64_t orow = o + (int64_t)r * XS;
-			if (xhi)
-				xp = vec(0.f);
-			else if (lane + SG >= wave_lanes)
-				xp = ld(orow + S);
-			if (xlo)
-				xm = vec(0.f);
-			else if (lane < SG)
-				xm = ld(orow - S);
+			if constexpr (prefetch)
+			{
+				// both edge loads issued before either is converted (one wait instead of two)
+				const bool ep = !xhi && lane + SG >= wave_lanes, em = !xlo && lane < SG;
+				raw rp, rm;
+				if (ep)
+					rp = draw<K, V>(dens, off + orow + S);
+				if (em)
+					rm = draw<K, V>(dens, off + orow - S);
+				if (xhi)
+					xp = vec(0.f);
+				else if (ep)
+					xp = dcvt<K, V>(rp) * is;
+				if (xlo)
+					xm = vec(0.f);
+				else if (em)
+					xm = dcvt<K, V>(rm) * is;
+			}
+			else
+			{
+				if (xhi)
+					xp = vec(0.f);
+				else if (lane + SG >= wave_lanes)
+					xp = ld(orow + S);
+				if (xlo)
+					xm = vec(0.f);
+				else if (lane < SG)
+					xm = ld(orow - S);
+			}
 			const vec ym = r == 0 ? below : c[r - 1 < 0 ? 0 : r - 1];
 			const vec yp = r == ny - 1 ? above : c[r + 1 < TY ? r + 1 : TY - 1];
 			vec gx, gy, gz;

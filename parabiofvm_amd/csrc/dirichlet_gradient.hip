@@ -609,7 +609,7 @@ __device__ __forceinline__ void gradient_tile_flat4(const gradient_args& a, int 
 // TY rows, and partly filled tiles at a row's end idle at most one wave-width of lanes.  Results are the same
 // expressions on the same values as gradient_tile, bit for bit.  Every wave runs every barrier, also waves
 // whose rows or columns lie outside the mesh.
-template <int V, int K, int TY>
+template <int V, int K, int TY, bool PF = false>
 __device__ __forceinline__ void gradient_tile_stacked(const gradient_args& a, int tile, typename vec_of<V>::type* sh)
 {
 	using vec = typename vec_of<V>::type;
@@ -656,11 +656,13 @@ __device__ __forceinline__ void gradient_tile_stacked(const gradient_args& a, in
 		zm[r] = (run && r < ny && z0 > 0) ? ld((int64_t)(z0 - 1) * plane + (int64_t)r * XS) : vec(0.f);
 		c[r] = (run && r < ny) ? ld((int64_t)z0 * plane + (int64_t)r * XS) : vec(0.f);
 	}
-	// 16-bit forms load plane z + 1 and the halo rows of plane z (raw) one plane step ahead, as the side-by-side
-	// tile does.  Loaded and converted in the same step, each conditional 16-bit load was waited for where it
-	// was issued (the conversion sits inside its branch), so the step's loads ran one after another: 400^3 x 8
-	// FP16 stacked tiles 2.05-2.42 ms against 1.46-1.75 ms for FP32.
-	constexpr bool prefetch = K != store_f32;
+	// PF (16-bit forms of the single-instance kernel): load plane z + 1 and the halo rows of plane z (raw) one
+	// plane step ahead, as the side-by-side tile does.  Loaded and converted in the same step, each conditional
+	// 16-bit load was waited for where it was issued (the conversion sits inside its branch), so the step's
+	// loads ran one after another: 400^3 x 8 FP16 stacked tiles 2.05-2.42 ms, 1.45 ms with PF.  The batched
+	// kernel keeps the form without: with PF (3 waves per SIMD instead of 4) the batch autotuner picked stacked
+	// tiles that ran slower inside the step (64-instance FP16 gradient 3.28 -> 3.50 ms).
+	constexpr bool prefetch = PF && K != store_f32;
 	using raw = typename svec_of<K, V>::type;
 	raw pzp[prefetch ? TY : 1], pbelow, pabove;
 	const bool need_below = run && !below_wave && y0 > 0;
@@ -820,7 +822,7 @@ __global__ __launch_bounds__(grad_tile) void gradient3d_kernel(gradient_args a, 
 	if constexpr (STACKED)
 	{
 		__shared__ typename vec_of<V>::type sh[2 * 2 * 4 * wave_lanes];
-		gradient_tile_stacked<V, K, gradient_rows<V, K>()>(a, tile, sh);
+		gradient_tile_stacked<V, K, gradient_rows<V, K>(), true>(a, tile, sh);
 	}
 	else
 		gradient_tile<V, K, gradient_rows<V, K>()>(a, tile);

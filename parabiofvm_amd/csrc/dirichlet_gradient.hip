@@ -609,6 +609,11 @@ __device__ __forceinline__ void gradient_tile_flat4(const gradient_args& a, int 
 // TY rows, and partly filled tiles at a row's end idle at most one wave-width of lanes.  Results are the same
 // expressions on the same values as gradient_tile, bit for bit.  Every wave runs every barrier, also waves
 // whose rows or columns lie outside the mesh.
+// Halo lanes (16-bit storage, 4 substrates per lane): the S/4 lanes at each end of a wave load the columns
+// next to the wave's segment and store nothing, so every x neighbour comes from a lane of the wave and no
+// lane loads one by itself.  400^3 x 8 FP16: 1.45 -> 1.38 ms.  In FP32 the same mapping ran 1.46 -> 1.57 ms
+// (60-lane segments start off the 128-byte lines, and FP32 rows are twice as long), so FP32 keeps 64-lane
+// segments and loads the two edge columns of each row.  Needs S / 4 < 32 (stacked_gradient_ok).
 template <int V, int K, int TY, bool PF = false>
 __device__ __forceinline__ void gradient_tile_stacked(const gradient_args& a, int tile, typename vec_of<V>::type* sh)
 {
@@ -618,11 +623,14 @@ __device__ __forceinline__ void gradient_tile_stacked(const gradient_args& a, in
 	const int wave = threadIdx.x / wave_lanes;
 	const int lane = threadIdx.x & (wave_lanes - 1);
 	const int y0 = (row_tile / a.tiles_per_row) * (4 * TY) + wave * TY;
-	const int q = ((row_tile % a.tiles_per_row) * wave_lanes + lane) * V;
+	constexpr bool halo = V == 4 && K != store_f32;
+	const int SGh = halo ? a.S / V : 0; // halo lanes at each end of the wave
+	const int q = ((row_tile % a.tiles_per_row) * (wave_lanes - 2 * SGh) + lane - SGh) * V;
 	const int S = a.S, XS = a.XS, Y = a.Y, Z = a.Z;
-	const bool col = q < XS;
+	const bool col = q >= 0 && q < XS;
 	const int ny = max(0, min(TY, Y - y0)); // rows of this wave (0: outside the mesh, barriers only)
-	const bool run = col && ny > 0;
+	const bool run = col && ny > 0; // loads
+	const bool outp = run && lane >= SGh && lane < wave_lanes - SGh; // computes and stores
 	const int z0 = zc * a.zchunk;
 	const int z1 = min(Z, z0 + a.zchunk);
 	const int64_t plane = (int64_t)Y * XS;
@@ -665,8 +673,8 @@ __device__ __forceinline__ void gradient_tile_stacked(const gradient_args& a, in
 	constexpr bool prefetch = PF && K != store_f32;
 	using raw = typename svec_of<K, V>::type;
 	raw pzp[prefetch ? TY : 1], pbelow, pabove;
-	const bool need_below = run && !below_wave && y0 > 0;
-	const bool need_above = run && !above_wave && y0 + ny < Y;
+	const bool need_below = outp && !below_wave && y0 > 0;
+	const bool need_above = outp && !above_wave && y0 + ny < Y;
 	auto fetch = [&](int zz) {
 		const int64_t oo = (int64_t)zz * plane;
 #pragma unroll
@@ -726,11 +734,18 @@ __device__ __forceinline__ void gradient_tile_stacked(const gradient_args& a, in
 			// x neighbours across lanes (every lane of the wave takes part in the exchange)
 			vec xp = shfl_vec<V>(c[r], lane + SG);
 			vec xm = shfl_vec<V>(c[r], lane - SG);
-			if (!run || r >= ny)
+			if (!outp || r >= ny)
 				continue;
 			const int y = y0 + r;
 			const int64_t orow = o + (int64_t)r * XS;
-			if constexpr (prefetch)
+			if constexpr (halo)
+			{
+				if (xhi)
+					xp = vec(0.f);
+				if (xlo)
+					xm = vec(0.f);
+			}
+			else if constexpr (prefetch)
 			{
 				// both edge loads issued before either is converted (one wait instead of two)
 				const bool ep = !xhi && lane + SG >= wave_lanes, em = !xlo && lane < SG;
@@ -899,6 +914,20 @@ void launch_copy(bfvm_ctx* ctx)
 	BFVM_HIP(hipGetLastError());
 }
 
+// Stacked tiles in 16-bit storage with 4 substrates per lane give S / 4 lanes at each end of a wave to halo
+// columns, so they need S / 4 < 32; otherwise the side-by-side layout is used
+static bool stacked_gradient_ok(const bfvm_ctx* ctx)
+{
+	return ctx->storage == BFVM_STORAGE_F32 || ctx->mesh.S % 4 != 0 || 2 * (ctx->mesh.S / 4) < wave_lanes;
+}
+static bool stacked_gradient_ok(const bfvm_batch* b)
+{
+	for (const bfvm_ctx* c : b->ctx)
+		if (!stacked_gradient_ok(c))
+			return false;
+	return true;
+}
+
 // Tiling of one 3-D field with `rows` y rows per thread: z is split so that there are about `target` tiles
 // (workgroups).  Stacked tiles: 64 columns by 4 * rows rows per workgroup; otherwise 256 columns by `rows`.
 // flat: one-element instances take gradient_tile_flat4 (vec = flat4_vec, 4 row positions per thread)
@@ -915,7 +944,11 @@ static gradient_args make_gradient_args(const bfvm_ctx* ctx, int64_t target, int
 	a.S = m.S;
 	a.XS = m.grid[0] * m.S;
 	a.vec = m.S % 4 == 0 ? 4 : (flat && !stacked ? flat4_vec : 1);
-	const int width = stacked ? wave_lanes : grad_tile;
+	// stacked 16-bit 4-substrate tiles: S / 4 halo lanes at each end of a wave
+	const int halo = stacked && a.vec == 4 && ctx->storage != BFVM_STORAGE_F32 ? m.S / 4 : 0;
+	if (2 * halo >= wave_lanes)
+		throw std::logic_error("stacked gradient tiles: too many substrates for the halo lanes");
+	const int width = stacked ? wave_lanes - 2 * halo : grad_tile;
 	const int band = stacked ? 4 * rows : rows;
 	const int per_lane = a.vec == flat4_vec ? 4 : a.vec;
 	a.tiles_per_row = ((a.XS + per_lane - 1) / per_lane + width - 1) / width;
@@ -963,7 +996,7 @@ void prepare_batch_gradient(bfvm_batch* b)
 		}
 		// tiles of `zc` z planes: ask for (row tiles) x ceil(Z / zc)
 		const int rows = b->tuning.gradient_rows, zc = std::max(1, b->tuning.gradient_zchunk);
-		const bool stacked = b->tuning.gradient_stacked != 0;
+		const bool stacked = b->tuning.gradient_stacked != 0 && stacked_gradient_ok(b);
 		const bool flat = true; // one-element instances as 4 row positions per thread
 		const gradient_args probe = make_gradient_args(ctx, 1, rows, stacked, flat);
 		const int64_t row_tiles = (int64_t)probe.ntiles;
@@ -1020,7 +1053,7 @@ static void launch_batched_gradient_t(bfvm_batch* b)
 
 void launch_batched_gradient(bfvm_batch* b)
 {
-	const bool st = b->tuning.gradient_stacked != 0;
+	const bool st = b->tuning.gradient_stacked != 0 && stacked_gradient_ok(b);
 	if (b->tuning.gradient_rows == 8)
 		st ? launch_batched_gradient_t<8, true>(b) : launch_batched_gradient_t<8, false>(b);
 	else
@@ -1043,7 +1076,7 @@ void launch_gradient(bfvm_ctx* ctx)
 		by_storage(ctx->storage, [&](auto k) {
 			constexpr int K = decltype(k)::value;
 			const bool v4 = m.S % 4 == 0;
-			const bool st = ctx->gradient_stacked;
+			const bool st = ctx->gradient_stacked && stacked_gradient_ok(ctx);
 			const gradient_args a =
 				make_gradient_args(ctx, target, v4 ? gradient_rows<4, K>() : gradient_rows<1, K>(), st);
 			const int per_xcd = (a.ntiles + num_xcds - 1) / num_xcds;

@@ -255,13 +255,15 @@ def test_gpu_gradient_bitwise(name, dims, shape, S, pkg, oracle_mod):
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])
 @pytest.mark.parametrize("shape,S", [((3, 4, 5), 2), ((70, 45, 33), 3), ((64, 40, 36), 8), ((100, 37, 9), 16),
-                                     ((17, 2, 9), 4), ((130, 71, 6), 1)])
+                                     ((17, 2, 9), 4), ((130, 71, 6), 1), ((7, 9, 5), 124), ((9, 6, 5), 128)])
 def test_gpu_gradient_stacked_tiles(shape, S, storage, pkg, oracle_mod):
     """The gradient's stacked tile layout (waves stacked along y, the rows where they meet passed through LDS) on
     meshes whose rows and columns do not fill whole tiles: bit-identical to the float oracle (FP32 storage) and
     to the other tile layout (FP16 storage), at several workgroup targets (z split into chunks of different
     lengths, so tiles start and end at every plane parity), in both workgroup orders (the plane order covers
-    every tile exactly once: the output is NaN-filled before each launch)."""
+    every tile exactly once: the output is NaN-filled before each launch).  16-bit stacked tiles give S / 4
+    lanes at each end of a wave to halo columns: 124 substrates leave 2 computing lanes per wave, and at 128
+    the launch takes the side-by-side layout instead."""
     p = random_problem(3, shape, S, seed=45)
     o, g = build_pair(pkg, oracle_mod, p)
     if storage != "f32":

@@ -351,6 +351,27 @@ __device__ __forceinline__ void gradient_tile(const gradient_args& a, int tile)
 				xp = ld(xhi ? orow : orow + S);
 				xm = ld(xlo ? orow : orow - S);
 			}
+			else if constexpr (K != store_f32)
+			{
+				// 16-bit: both edge loads issued before either is converted (converted inside its own branch,
+				// each load was waited for where it was issued)
+				xp = shfl_vec<V>(c[r], lane + SG);
+				xm = shfl_vec<V>(c[r], lane - SG);
+				const bool ep = !xhi && lane + SG >= wave_lanes, em = !xlo && lane < SG;
+				raw rp, rm;
+				if (ep)
+					rp = draw<K, V>(dens, off + orow + S);
+				if (em)
+					rm = draw<K, V>(dens, off + orow - S);
+				if (xhi)
+					xp = vec(0.f);
+				else if (ep)
+					xp = dcvt<K, V>(rp) * is;
+				if (xlo)
+					xm = vec(0.f);
+				else if (em)
+					xm = dcvt<K, V>(rm) * is;
+			}
 			else
 			{
 				xp = shfl_vec<V>(c[r], lane + SG);

@@ -541,6 +541,24 @@ __device__ __forceinline__ void gradient_tile_flat4(const gradient_args& a, int 
 		return r;
 	};
 	auto cvt = [&](raw r) -> vec { return __builtin_convertvector(r, vec) * is; };
+	// ldx in two halves for 16-bit forms: both x-neighbour loads of a row are issued before either is
+	// converted (converted inside its branch, each load was waited for where it was issued)
+	auto rldx = [&](int64_t o, int d) -> raw {
+		raw r;
+		if (inner)
+			r = gload((const uv*)(dens + o + d));
+		else
+		{
+#pragma unroll
+			for (int c = 0; c < 4; c++)
+			{
+				const int p = q + c + d;
+				const bool in = q + c < XS && p >= 0 && p < XS;
+				r[c] = in ? gload(dens + o + c + d) : (elem)0.f;
+			}
+		}
+		return r;
+	};
 	auto fetch = [&](int zz) {
 		const int64_t oo = (int64_t)zz * plane;
 #pragma unroll
@@ -585,8 +603,18 @@ __device__ __forceinline__ void gradient_tile_flat4(const gradient_args& a, int 
 				break;
 			const int y = y0 + r;
 			const int64_t orow = o + (int64_t)r * XS;
-			const vec xp = ldx(orow, S);
-			const vec xm = ldx(orow, -S);
+			vec xp, xm;
+			if constexpr (K == store_f32)
+			{
+				xp = ldx(orow, S);
+				xm = ldx(orow, -S);
+			}
+			else
+			{
+				const raw rp = rldx(orow, S), rm = rldx(orow, -S);
+				xp = cvt(rp);
+				xm = cvt(rm);
+			}
 			const vec ym = r == 0 ? below : cc[r - 1 < 0 ? 0 : r - 1];
 			const vec yp = r == ny - 1 ? above : cc[r + 1 < TY ? r + 1 : TY - 1];
 			vec gx, gy, gz;

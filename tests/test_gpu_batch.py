@@ -279,3 +279,48 @@ def test_batch_agents_change_between_steps(pkg, oracle_mod):
         tag = f"instance {k} grid {cfg['grid']} S {cfg['S']}"
         assert_close(g.densities, o.densities, f"{tag} densities")
     batch.close()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+@pytest.mark.parametrize("rows", [4, 8])
+def test_batch_stacked_gradient_tiles(storage, rows, pkg, oracle_mod, tmp_path):
+    """The batched gradient with stacked tiles, which the batch autotuner need not pick: forced through a
+    tuning-cache entry for this batch's shapes (applied by bfvm_batch_autotune), on 4-substrate instances (16-bit:
+    with halo lanes) and one-element instances, with 4 and 8 rows per thread.  Each instance's gradient equals the
+    gradient of its densities as stored, and in FP32 storage the densities stay within the oracle's tolerance."""
+    from parabiofvm_amd import _native
+    from parabiofvm_amd.configs import apply_config, make_config
+
+    shapes = [((20, 12, 9), 4), ((11, 10, 8), 3), ((9, 14, 7), 8), ((13, 6, 10), 5)]
+    cfgs = [make_config(1, seed=300 + k, shape=g, substrates=S, agents=40) for k, (g, S) in enumerate(shapes)]
+    code = {"f32": 0, "bf16": 1, "f16": 2}[storage]
+    key = f"{code}:{len(shapes)}" + "".join(f";{g[0]},{g[1]},{g[2]},{S},3" for g, S in shapes)
+    line = f"batch {key} " + " ".join(["256"] * 6) + " " + " ".join(["2048"] * 6) + f" 1 4096 {rows} 16 0 1\n"
+    path = tmp_path / "tune.txt"
+    path.write_text(line)
+    _native.tuning_load(str(path))
+    oracles, solvers = [], []
+    for cfg in cfgs:
+        o = oracle_mod.Oracle(*args_of(cfg), dtype=np.float32)
+        g = pkg.DeviceSolver(*args_of(cfg))
+        g.set_storage(storage)
+        for be in (o, g):
+            apply_config(cfg, be)
+            be.initialize()
+        g.store()
+        oracles.append(o)
+        solvers.append(g)
+    batch = pkg.Batch(solvers)
+    batch.autotune()
+    steps = 3
+    for i in range(steps):
+        batch.timestep(i == 0, True)
+        for o in oracles:
+            o.timestep(i == 0, True)
+    batch.wait()
+    for g in solvers:
+        g.load()
+    check_gradients_of_stored(cfgs, solvers, f"stacked rows {rows} {storage}")
+    if storage == "f32":
+        check_instances(cfgs, oracles, solvers, steps)
+    batch.close()

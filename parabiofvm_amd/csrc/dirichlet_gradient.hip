@@ -451,7 +451,6 @@ __device__ __forceinline__ void gradient_tile_flat4(const gradient_args& a, int 
 	if (q >= XS)
 		return;
 	const bool whole = q + 4 <= XS;
-	const bool inner = whole && q >= S && q + 4 + S <= XS;
 	const int z0 = zc * a.zchunk;
 	const int z1 = min(Z, z0 + a.zchunk);
 	const int ny = min(TY, Y - y0);
@@ -469,10 +468,19 @@ __device__ __forceinline__ void gradient_tile_flat4(const gradient_args& a, int 
 		for (int c = 0; c < 4; c++)
 			is[c] = a.inv[(q + c) % S];
 	}
+	// Loads take 4 consecutive elements as one vector wherever the 4 lie inside the array (vin), also where
+	// they cross the row's start or end: the elements of the neighbouring row feed only components that are not
+	// stored (past the row's end) or x neighbours that one-sided differences do not use (x at a boundary).  The
+	// per-component form remains for the array's first and last elements.  Tested per lane on the row's
+	// bounds (whole, inner), nearly every wave held a lane at a row's end and ran both forms.
+	auto vin = [&](int64_t o) {
+		const int64_t e = off + o;
+		return e >= 0 && e + 4 <= a.VS;
+	};
 	// 4 positions from element o of this thread's row position; components past the row's end read as 0
 	auto ld = [&](int64_t o) -> vec {
 		vec r;
-		if (whole)
+		if (vin(o))
 			r = __builtin_convertvector(gload((const uv*)(dens + o)), vec);
 		else
 		{
@@ -489,7 +497,7 @@ __device__ __forceinline__ void gradient_tile_flat4(const gradient_args& a, int 
 	// a boundary, where the one-sided difference does not use it) read their own value
 	auto ldx = [&](int64_t o, int d) -> vec {
 		vec r;
-		if (inner)
+		if (vin(o + d))
 			r = __builtin_convertvector(gload((const uv*)(dens + o + d)), vec);
 		else
 		{
@@ -530,7 +538,7 @@ __device__ __forceinline__ void gradient_tile_flat4(const gradient_args& a, int 
 	raw pzp[prefetch ? TY : 1], pbelow, pabove;
 	auto rld = [&](int64_t o) -> raw {
 		raw r;
-		if (whole)
+		if (vin(o))
 			r = gload((const uv*)(dens + o));
 		else
 		{
@@ -545,7 +553,7 @@ __device__ __forceinline__ void gradient_tile_flat4(const gradient_args& a, int 
 	// converted (converted inside its branch, each load was waited for where it was issued)
 	auto rldx = [&](int64_t o, int d) -> raw {
 		raw r;
-		if (inner)
+		if (vin(o + d))
 			r = gload((const uv*)(dens + o + d));
 		else
 		{

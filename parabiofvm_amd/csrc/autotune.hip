@@ -412,7 +412,7 @@ void autotune_batch(bfvm_batch* b)
 		int best_tiles = b->tuning.gradient_tiles, best_blocks = b->tuning.gradient_blocks;
 		float best_t = 1e30f;
 		for (int tiles : { 1, 2, 4, 8 })
-			for (int blocks : { 2048, 4096, 8192, 16384 })
+			for (int blocks : { 2048, 4096, 8192, 16384, 32768, 65536 })
 			{
 				b->tuning.gradient_tiles = tiles;
 				b->tuning.gradient_blocks = blocks;
